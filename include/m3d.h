@@ -1,0 +1,153 @@
+/*
+ * m3d.h -- C-ABI of the MI355X FFT homogenization solver (libmat3d.so).
+ *
+ * Drop-in boundary for the reference's two solver classes.  Each entry point
+ * below replaces one public method; the reference interface it stands for is
+ * cited as file:line relative to the reference tree.  Plain pointers and
+ * sizes only; arrays use the reference's memory order so Eigen callers pass
+ * .data() unchanged:
+ *   - real fields: C x N column-major (C = 6 elastic, 3 thermal), i.e. the
+ *     C components of voxel i are contiguous, voxel i = z + Nz*(y + Ny*x);
+ *   - material tables: nph x 7 column-major [E, nu, sigma_y, H, n, P, Y0]
+ *     (example/3d/main.cpp:104-111);
+ *   - 6x6 / 3x3 results: column-major.
+ *
+ * Errors: every function returns M3D_OK (0) or a negative M3D_ERR_* code and
+ * never exits, throws or aborts across the ABI; m3d_*_last_error() returns
+ * a message.  Handles are independent (no global cleanup; compare the
+ * reference destructor's fftw_cleanup, src/3d/FFT_analysis3D.cpp:1141-1142).
+ * One handle must not be used from two threads at once.
+ */
+#ifndef M3D_H
+#define M3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3D_OK 0
+#define M3D_ERR_ARG -1         /* invalid argument (sizes, phase ids, behaviour) */
+#define M3D_ERR_UNSUPPORTED -2 /* grid size with a prime factor > 13, > 256 phases */
+#define M3D_ERR_OOM -3         /* device allocation failed */
+#define M3D_ERR_HIP -4         /* HIP runtime error */
+#define M3D_ERR_STATE -5       /* call out of order (e.g. run before material) */
+#define M3D_ERR_NUMERIC -6     /* singular 6x6 / 3x3 or non-finite state */
+
+/* precision policy */
+#define M3D_FP64 0            /* fields, FFTs and reductions in FP64 */
+#define M3D_FP32_POLISH 1     /* FP32 fields/FFTs, FP64 reductions, FP64 polish to the stop rule */
+
+typedef struct m3d_elastic m3d_elastic;
+typedef struct m3d_thermal m3d_thermal;
+
+/* per-solve outcome of a run (no counterpart in the reference, which prints
+ * Itter/eqVar/diff_sum_field to stdout every iteration: FFT_analysis3D.cpp:394-412) */
+typedef struct {
+    int iterations;     /* itNo at exit */
+    int status;         /* 1 stop rule met, 2 iteration cap reached */
+    int iters_fp32;     /* iterations executed in FP32 (0 under M3D_FP64) */
+    int iters_fp64;     /* iterations executed in FP64 */
+    double eqVar;       /* last evaluated avr_eqVar */
+    double diff_sum;    /* last diff_sum_field */
+    double diff_eq;     /* last diff_avr_eqVar (only evaluated when diff_sum < tol_field) */
+    double time_ms;     /* device time of the solve loop */
+} m3d_run_info;
+
+/* ---------------------------------------------------------------- elastic */
+
+/* FFT_Analysis3D(int Nx, int Ny, int Nz, string mater_behav)
+ * -- include/FFT_analysis3D.hpp:196, src/3d/FFT_analysis3D.cpp:25-152.
+ * behav: "e" (accelerated scheme); "ep"/"ed" are declared by the reference
+ * but not provided by this library yet (M3D_ERR_UNSUPPORTED). */
+int m3d_elastic_create(int nx, int ny, int nz, const char *behav, int precision, m3d_elastic **out);
+
+/* ~FFT_Analysis3D() -- src/3d/FFT_analysis3D.cpp:1107-1147 */
+int m3d_elastic_destroy(m3d_elastic *h);
+
+/* void material(MatrixDdd& mat, MatrixIdd& RVE, int noPhases)
+ * -- include/FFT_analysis3D.hpp:200, src/3d/FFT_analysis3D.cpp:420-549.
+ * mat: nph x 7 column-major; rve: N phase ids in [0, nph). */
+int m3d_elastic_material(m3d_elastic *h, const double *mat, const int32_t *rve, int nph);
+
+/* void setLoad(MatrixDdd& E, MatrixDdd& X0)
+ * -- include/FFT_analysis3D.hpp:202, src/3d/FFT_analysis3D.cpp:551-564. */
+int m3d_elastic_set_load(m3d_elastic *h, const double *E, const double *X0);
+
+/* void runAnalysis(double tol_field, double tol_eq, int nInc)
+ * -- include/FFT_analysis3D.hpp:206, src/3d/FFT_analysis3D.cpp:749-854.
+ * info may be NULL. */
+int m3d_elastic_run(m3d_elastic *h, double tol_field, double tol_eq, int nInc, m3d_run_info *info);
+
+/* double homogenization(double tol_field, double tol_eq, string out)
+ * -- include/FFT_analysis3D.hpp:209, src/3d/FFT_analysis3D.cpp:856-999.
+ * The six load cases run as one batched launch.  Outputs (column-major
+ * 6x6, any may be NULL): CHomo, SHomo, epHomo, sigHomo; *ret =
+ * 1/S11 + 1/S22 + 1/S33 (the reference's return value); info6: 6 entries. */
+int m3d_elastic_homogenization(m3d_elastic *h, double tol_field, double tol_eq, double *C, double *S,
+                               double *epHomo, double *sigHomo, double *ret, m3d_run_info *info6);
+
+/* public members epsilon / sigma (6 x N) -- include/FFT_analysis3D.hpp:98-99.
+ * Copies the fields of the last run (for homogenization: load case `lc`,
+ * 0..5; for run: lc = 0) in the reference's memory order. */
+int m3d_elastic_get_fields(m3d_elastic *h, int lc, double *eps, double *sig);
+
+/* public member vol_fractions and the chosen algorithm (strain/stress base),
+ * include/FFT_analysis3D.hpp:58-59,141.  vol: nph entries (may be NULL);
+ * strain_base: 1 strain-based, 0 stress-based. */
+int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0, double *mu0);
+
+/* Tuning knobs (no reference counterpart): key "polish_factor" (FP32 phase
+ * stops at tol*factor, default 10), "cap" (iteration cap, default 2000),
+ * "chunk" (iterations per graph replay). */
+int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
+
+const char *m3d_elastic_last_error(m3d_elastic *h);
+
+/* ---------------------------------------------------------------- thermal */
+
+/* FFT_analysis3d_thermal(int Nx, int Ny, int Nz)
+ * -- include/FFT_thermal3D.hpp:133, src/3dTherm/FFT_thermal3D.cpp:14-91 */
+int m3d_thermal_create(int nx, int ny, int nz, int precision, m3d_thermal **out);
+
+int m3d_thermal_destroy(m3d_thermal *h);
+
+/* void material(double km, double ki, MatrixIdd& RVE)
+ * -- include/FFT_thermal3D.hpp:138, src/3dTherm/FFT_thermal3D.cpp:373-422.
+ * RVE != 0 -> matrix (km), RVE == 0 -> inclusion (ki). */
+int m3d_thermal_material(m3d_thermal *h, double km, double ki, const int32_t *rve);
+
+/* N-phase extension of material(): conductivity k[p] for phase id p.  No
+ * reference counterpart (the reference supports two phases only,
+ * FFT_thermal3D.cpp:186-189); rule: k0 = sqrt(kmin*kmax), gradient base iff
+ * the first max-k phase has the largest volume fraction. */
+int m3d_thermal_material_n(m3d_thermal *h, const double *k, int nph, const int32_t *rve);
+
+/* void setLoad(MatrixDdd& E) -- include/FFT_thermal3D.hpp:140, FFT_thermal3D.cpp:426-441 */
+int m3d_thermal_set_load(m3d_thermal *h, const double *E);
+
+/* void runAnalysis(double tol_field, double tol_eq)
+ * -- include/FFT_thermal3D.hpp:145, FFT_thermal3D.cpp:446-485 */
+int m3d_thermal_run(m3d_thermal *h, double tol_field, double tol_eq, m3d_run_info *info);
+
+/* void homogenization(double, double, string) -- include/FFT_thermal3D.hpp:148,
+ * FFT_thermal3D.cpp:489-622.  Outputs 3x3 column-major: KHomo, RHomo,
+ * tempGradHomo, heatFluxHomo (any may be NULL); info3: 3 entries. */
+int m3d_thermal_homogenization(m3d_thermal *h, double tol_field, double tol_eq, double *K, double *R,
+                               double *tempGradHomo, double *heatFluxHomo, m3d_run_info *info3);
+
+/* public members tempGradient / heatFlux (3 x N) -- include/FFT_thermal3D.hpp:82,92 */
+int m3d_thermal_get_fields(m3d_thermal *h, int lc, double *grad, double *flux);
+
+int m3d_thermal_set_option(m3d_thermal *h, const char *key, double value);
+
+const char *m3d_thermal_last_error(m3d_thermal *h);
+
+/* library version string */
+const char *m3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* M3D_H */

@@ -1,0 +1,274 @@
+"""mat3d_amd -- MI355X-native 3D FFT homogenization (Moulinec-Suquet / Eyre-Milton).
+
+Python mirror of the reference's solver classes over the C-ABI of
+``libmat3d.so`` (declared in include/m3d.h):
+
+  * ``FFT_Analysis3D``         -- include/FFT_analysis3D.hpp:36-215
+  * ``FFT_analysis3d_thermal`` -- include/FFT_thermal3D.hpp:37-154
+
+Method names, argument order and meaning follow the reference.  Array
+conventions are the reference's: real fields are ``(N, C)`` voxel-major
+(Eigen ``C x N`` column-major), voxel index ``i = z + Nz*(y + Ny*x)``;
+material tables are ``nph x 7`` ``[E, nu, sigma_y, H, n, P, Y0]``.
+
+The compute path is the HIP library only: importing a solver without a
+built ``libmat3d.so`` raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmat3d.so")
+
+M3D_OK = 0
+ERRORS = {-1: "M3D_ERR_ARG", -2: "M3D_ERR_UNSUPPORTED", -3: "M3D_ERR_OOM", -4: "M3D_ERR_HIP",
+          -5: "M3D_ERR_STATE", -6: "M3D_ERR_NUMERIC"}
+PRECISION = {"fp64": 0, "fp32_polish": 1}
+
+# every symbol include/m3d.h declares
+EXPORTS = [
+    "m3d_version",
+    "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
+    "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
+    "m3d_elastic_set_option", "m3d_elastic_last_error",
+    "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
+    "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
+    "m3d_thermal_set_option", "m3d_thermal_last_error",
+]
+
+
+class RunInfo(ct.Structure):
+    _fields_ = [("iterations", ct.c_int), ("status", ct.c_int), ("iters_fp32", ct.c_int), ("iters_fp64", ct.c_int),
+                ("eqVar", ct.c_double), ("diff_sum", ct.c_double), ("diff_eq", ct.c_double),
+                ("time_ms", ct.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class M3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libmat3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mat3d_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ct.CDLL(LIB_PATH)
+        vp, ip, dp, cp = ct.c_void_p, ct.c_int, ct.POINTER(ct.c_double), ct.c_char_p
+        i32p = ct.POINTER(ct.c_int32)
+        rip = ct.POINTER(RunInfo)
+        ipp = ct.POINTER(ct.c_int)
+        sig = {
+            "m3d_version": (cp, []),
+            "m3d_elastic_create": (ip, [ip, ip, ip, cp, ip, ct.POINTER(vp)]),
+            "m3d_elastic_destroy": (ip, [vp]),
+            "m3d_elastic_material": (ip, [vp, dp, i32p, ip]),
+            "m3d_elastic_set_load": (ip, [vp, dp, dp]),
+            "m3d_elastic_run": (ip, [vp, ct.c_double, ct.c_double, ip, rip]),
+            "m3d_elastic_homogenization": (ip, [vp, ct.c_double, ct.c_double, dp, dp, dp, dp, dp, rip]),
+            "m3d_elastic_get_fields": (ip, [vp, ip, dp, dp]),
+            "m3d_elastic_info": (ip, [vp, dp, ipp, dp, dp]),
+            "m3d_elastic_set_option": (ip, [vp, cp, ct.c_double]),
+            "m3d_elastic_last_error": (cp, [vp]),
+            "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
+            "m3d_thermal_destroy": (ip, [vp]),
+            "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
+            "m3d_thermal_material_n": (ip, [vp, dp, ip, i32p]),
+            "m3d_thermal_set_load": (ip, [vp, dp]),
+            "m3d_thermal_run": (ip, [vp, ct.c_double, ct.c_double, rip]),
+            "m3d_thermal_homogenization": (ip, [vp, ct.c_double, ct.c_double, dp, dp, dp, dp, rip]),
+            "m3d_thermal_get_fields": (ip, [vp, ip, dp, dp]),
+            "m3d_thermal_set_option": (ip, [vp, cp, ct.c_double]),
+            "m3d_thermal_last_error": (cp, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _i32p(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_int32))
+
+
+def _colmajor(m):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float64).T).reshape(-1)
+
+
+class FFT_Analysis3D:
+    """Mirror of ``FFT_Analysis3D`` (include/FFT_analysis3D.hpp:36-215) on the HIP solver."""
+
+    def __init__(self, Nx1, Ny1, Nz1, mater_behav="e", precision="fp64"):
+        self.Nx, self.Ny, self.Nz = int(Nx1), int(Ny1), int(Nz1)
+        self.xyz = self.Nx * self.Ny * self.Nz
+        self.mater_behav = mater_behav
+        h = ct.c_void_p()
+        rc = lib().m3d_elastic_create(self.Nx, self.Ny, self.Nz, mater_behav.encode(), PRECISION[precision],
+                                      ct.byref(h))
+        if rc != M3D_OK:
+            raise M3DError(rc, "m3d_elastic_create failed")
+        self._h = h
+        self.last_info = None
+        self.info = []
+
+    def _chk(self, rc):
+        if rc != M3D_OK:
+            raise M3DError(rc, lib().m3d_elastic_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().m3d_elastic_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_option(self, key, value):
+        self._chk(lib().m3d_elastic_set_option(self._h, key.encode(), float(value)))
+
+    def material(self, mat, RVE1, noPhases1=None):
+        mat = np.asarray(mat, dtype=np.float64)
+        nph = mat.shape[0] if noPhases1 is None else int(noPhases1)
+        rve = np.ascontiguousarray(np.asarray(RVE1).reshape(-1), dtype=np.int32)
+        if rve.size != self.xyz:
+            raise ValueError("RVE size does not match the grid")
+        m = _colmajor(mat[:nph])
+        self._chk(lib().m3d_elastic_material(self._h, _dp(m), _i32p(rve), nph))
+        self.noPhases = nph
+        vol = np.zeros(nph)
+        sb = ct.c_int(0)
+        la0, mu0 = ct.c_double(0), ct.c_double(0)
+        self._chk(lib().m3d_elastic_info(self._h, _dp(vol), ct.byref(sb), ct.byref(la0), ct.byref(mu0)))
+        self.vol_fractions = vol
+        self.strainBase = bool(sb.value)
+        self.stressBase = not self.strainBase
+        self.la0, self.mu0 = la0.value, mu0.value
+
+    def setLoad(self, E1, X01=None):
+        E = np.ascontiguousarray(np.asarray(E1, dtype=np.float64).reshape(6))
+        X0 = E if X01 is None else np.ascontiguousarray(np.asarray(X01, dtype=np.float64).reshape(6))
+        self._chk(lib().m3d_elastic_set_load(self._h, _dp(E), _dp(X0)))
+        self.E, self.X0 = E.copy(), X0.copy()
+
+    def runAnalysis(self, tol_field, tol_eq, nInc1=1):
+        info = RunInfo()
+        self._chk(lib().m3d_elastic_run(self._h, float(tol_field), float(tol_eq), int(nInc1), ct.byref(info)))
+        self.last_info = info.as_dict()
+        self.info = [self.last_info]
+        return self.last_info
+
+    def homogenization(self, tol_field, tol_eq, outputName=None):
+        C, S, ep, sg = (np.zeros(36) for _ in range(4))
+        ret = ct.c_double(0)
+        infos = (RunInfo * 6)()
+        self._chk(lib().m3d_elastic_homogenization(self._h, float(tol_field), float(tol_eq), _dp(C), _dp(S), _dp(ep),
+                                                   _dp(sg), ct.byref(ret), infos))
+        f = lambda a: a.reshape(6, 6).T.copy()
+        self.CHomo, self.SHomo, self.epHomo, self.sigHomo = f(C), f(S), f(ep), f(sg)
+        self.E11, self.E22, self.E33 = 1 / self.SHomo[0, 0], 1 / self.SHomo[1, 1], 1 / self.SHomo[2, 2]
+        self.info = [i.as_dict() for i in infos]
+        return ret.value
+
+    def fields(self, lc=0):
+        eps = np.empty((self.xyz, 6))
+        sig = np.empty((self.xyz, 6))
+        self._chk(lib().m3d_elastic_get_fields(self._h, int(lc), _dp(eps), _dp(sig)))
+        return eps, sig
+
+    @property
+    def epsilon(self):
+        return self.fields(0)[0]
+
+    @property
+    def sigma(self):
+        return self.fields(0)[1]
+
+
+class FFT_analysis3d_thermal:
+    """Mirror of ``FFT_analysis3d_thermal`` (include/FFT_thermal3D.hpp:37-154) on the HIP solver."""
+
+    def __init__(self, Nx1, Ny1, Nz1, precision="fp64"):
+        self.Nx, self.Ny, self.Nz = int(Nx1), int(Ny1), int(Nz1)
+        self.xyz = self.Nx * self.Ny * self.Nz
+        h = ct.c_void_p()
+        rc = lib().m3d_thermal_create(self.Nx, self.Ny, self.Nz, PRECISION[precision], ct.byref(h))
+        if rc != M3D_OK:
+            raise M3DError(rc, "m3d_thermal_create failed")
+        self._h = h
+        self._E = None
+        self.info = []
+
+    def _chk(self, rc):
+        if rc != M3D_OK:
+            raise M3DError(rc, lib().m3d_thermal_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().m3d_thermal_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_option(self, key, value):
+        self._chk(lib().m3d_thermal_set_option(self._h, key.encode(), float(value)))
+
+    def material(self, km, ki, RVE1):
+        rve = np.ascontiguousarray(np.asarray(RVE1).reshape(-1), dtype=np.int32)
+        self._chk(lib().m3d_thermal_material(self._h, float(km), float(ki), _i32p(rve)))
+
+    def material_n(self, k, RVE1):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        rve = np.ascontiguousarray(np.asarray(RVE1).reshape(-1), dtype=np.int32)
+        self._chk(lib().m3d_thermal_material_n(self._h, _dp(k), int(k.size), _i32p(rve)))
+
+    def setLoad(self, E1):
+        E = np.ascontiguousarray(np.asarray(E1, dtype=np.float64).reshape(3))
+        self._chk(lib().m3d_thermal_set_load(self._h, _dp(E)))
+
+    def runAnalysis(self, tol_field, tol_eq):
+        info = RunInfo()
+        self._chk(lib().m3d_thermal_run(self._h, float(tol_field), float(tol_eq), ct.byref(info)))
+        self.info = [info.as_dict()]
+        return self.info[0]
+
+    def homogenization(self, tol_field, tol_eq, filename=None):
+        K, R, G, Q = (np.zeros(9) for _ in range(4))
+        infos = (RunInfo * 3)()
+        self._chk(lib().m3d_thermal_homogenization(self._h, float(tol_field), float(tol_eq), _dp(K), _dp(R), _dp(G),
+                                                   _dp(Q), infos))
+        f = lambda a: a.reshape(3, 3).T.copy()
+        self.KHomo, self.RHomo, self.tempGradHomo, self.heatFluxHomo = f(K), f(R), f(G), f(Q)
+        self.K11, self.K22, self.K33 = self.KHomo[0, 0], self.KHomo[1, 1], self.KHomo[2, 2]
+        self.info = [i.as_dict() for i in infos]
+
+    def fields(self, lc=0):
+        g = np.empty((self.xyz, 3))
+        q = np.empty((self.xyz, 3))
+        self._chk(lib().m3d_thermal_get_fields(self._h, int(lc), _dp(g), _dp(q)))
+        return g, q
+
+    @property
+    def tempGradient(self):
+        return self.fields(0)[0]
+
+    @property
+    def heatFlux(self):
+        return self.fields(0)[1]

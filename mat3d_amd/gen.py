@@ -1,0 +1,179 @@
+"""Seeded synthetic microstructures for the homogenization workloads.
+
+Phase-id convention follows the reference (include/FFT_analysis3D.hpp:44-45):
+phase 1 = matrix, phase 0 = inclusion.  Arrays are returned as int32 of shape
+``(Nx, Ny, Nz)``; flattening in C order gives the reference's voxel index
+``i = z + Nz*(y + Ny*x)`` (FFT_analysis3D.cpp:115-123).  All geometry is
+periodic with minimum-image distances.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+
+
+def sha256(ids: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(ids, dtype=np.int32).tobytes()).hexdigest()
+
+
+def sphere65() -> np.ndarray:
+    """The reference fixture example/3d/rve65.txt: a centred sphere of radius 15
+    (phase 0 where |x-(32,32,32)|^2 <= 225) in a 65^3 matrix (phase 1)."""
+    g = np.arange(65) - 32
+    r2 = g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2
+    return np.where(r2 <= 225, 0, 1).astype(np.int32)
+
+
+def _wrap(d, n):
+    return (d + n // 2) % n - n // 2
+
+
+def _stamp_ball(ids, c, r, val):
+    n = np.array(ids.shape)
+    rr = int(np.ceil(r))
+    off = np.arange(-rr, rr + 1)
+    ox, oy, oz = np.meshgrid(off, off, off, indexing="ij")
+    m = (ox**2 + oy**2 + oz**2) <= r * r
+    ix = (c[0] + ox[m]) % n[0]
+    iy = (c[1] + oy[m]) % n[1]
+    iz = (c[2] + oz[m]) % n[2]
+    ids[ix, iy, iz] = val
+
+
+def spheres(n, radius, frac, seed, incl=0, matrix=1, max_tries=2_000_000):
+    """Random sequential addition of non-overlapping balls |d|^2 <= r^2 (integer
+    centres, periodic) until the inclusion fraction reaches ``frac``."""
+    rng = np.random.default_rng(seed)
+    ids = np.full((n, n, n), matrix, dtype=np.int32)
+    centres = np.zeros((0, 3), dtype=np.int64)
+    target = frac * n**3
+    count = 0
+    vol1 = int(((np.arange(-radius, radius + 1)[:, None, None] ** 2 + np.arange(-radius, radius + 1)[None, :, None] ** 2
+                 + np.arange(-radius, radius + 1)[None, None, :] ** 2) <= radius * radius).sum())
+    tries = 0
+    while count < target and tries < max_tries:
+        tries += 1
+        c = rng.integers(0, n, size=3)
+        if len(centres):
+            d = _wrap(centres - c, n)
+            if np.any((d * d).sum(1) <= (2 * radius) ** 2):
+                continue
+        centres = np.vstack([centres, c])
+        _stamp_ball(ids, c, radius, incl)
+        count += vol1
+    return ids
+
+
+def multi_spheres(n, groups, seed, matrix=1, max_tries=5_000_000):
+    """Non-overlapping balls of several phases; ``groups`` is a list of
+    (phase, fraction, rmin, rmax).  Radii are drawn uniformly (integer)."""
+    rng = np.random.default_rng(seed)
+    ids = np.full((n, n, n), matrix, dtype=np.int32)
+    cs = np.zeros((0, 3), dtype=np.int64)
+    rs = np.zeros((0,), dtype=np.int64)
+    for phase, frac, rmin, rmax in groups:
+        target = frac * n**3
+        got = 0
+        tries = 0
+        while got < target and tries < max_tries:
+            tries += 1
+            r = int(rng.integers(rmin, rmax + 1))
+            c = rng.integers(0, n, size=3)
+            if len(cs):
+                d = _wrap(cs - c, n)
+                if np.any((d * d).sum(1) <= (rs + r) ** 2):
+                    continue
+            cs = np.vstack([cs, c])
+            rs = np.append(rs, r)
+            _stamp_ball(ids, c, r, phase)
+            got += _ball_volume(r)
+    return ids
+
+
+def _ball_volume(r):
+    o = np.arange(-r, r + 1)
+    return int(((o[:, None, None] ** 2 + o[None, :, None] ** 2 + o[None, None, :] ** 2) <= r * r).sum())
+
+
+def fibres(n, radius, frac, seed, length=64, incl=0, matrix=1):
+    """Straight finite cylinders (radius ``radius``, length ``length``) with
+    axes uniform on S^2 and centres uniform in the box, overlaps allowed,
+    added until the inclusion fraction reaches ``frac``."""
+    rng = np.random.default_rng(seed)
+    ids = np.full((n, n, n), matrix, dtype=np.int32)
+    half = length / 2.0
+    target = frac * n**3
+    count = 0
+    while count < target:
+        for _ in range(32):
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            c = rng.uniform(0, n, size=3)
+            ext = np.abs(v) * half + radius + 1
+            lo = np.floor(c - ext).astype(int)
+            hi = np.ceil(c + ext).astype(int)
+            ax = [np.arange(lo[k], hi[k] + 1) for k in range(3)]
+            X, Y, Z = np.meshgrid(ax[0] - c[0], ax[1] - c[1], ax[2] - c[2], indexing="ij")
+            t = X * v[0] + Y * v[1] + Z * v[2]
+            d2 = X * X + Y * Y + Z * Z - t * t
+            m = (np.abs(t) <= half) & (d2 <= radius * radius)
+            ii = np.nonzero(m)
+            ids[(ax[0][ii[0]]) % n, (ax[1][ii[1]]) % n, (ax[2][ii[2]]) % n] = incl
+        count = int((ids == incl).sum())
+    return ids
+
+
+def voronoi(n, grains, seed):
+    """Periodic Voronoi polycrystal: phase id = index of the nearest seed
+    (minimum image).  Uses scipy's periodic k-d tree."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0, n, size=(grains, 3))
+    tree = cKDTree(pts, boxsize=n)
+    g = np.arange(n) + 0.0
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) % n
+    _, idx = tree.query(q, k=1)
+    return idx.reshape(n, n, n).astype(np.int32)
+
+
+# ---------------------------------------------------------------- workloads
+MAT_C1 = np.array([[400e9, 0.20, np.inf, 0, 1, 0, 0],   # phase 0: inclusion
+                   [70e9, 0.30, np.inf, 0, 1, 0, 0]])   # phase 1: matrix
+MAT_C2 = np.array([[300e9, 0.25, np.inf, 0, 1, 0, 0],   # phase 0: fibre
+                   [3e9, 0.35, np.inf, 0, 1, 0, 0]])    # phase 1: matrix
+MAT_C4 = np.array([[400e9, 0.20, np.inf, 0, 1, 0, 0],   # phase 0: elastic inclusion
+                   [70e9, 0.30, 100e6, 1e9, 1, 0, 0]])  # phase 1: J2 matrix
+
+
+def c1_rve(n=64):
+    return spheres(n, 6, 0.20, seed=1)
+
+
+def c2_rve(n=256):
+    return fibres(n, 4, 0.30, seed=2)
+
+
+def c4_rve(n=192):
+    return spheres(n, 8, 0.25, seed=4)
+
+
+def c5_rve(n=512):
+    return multi_spheres(n, [(2, 0.15, 3, 10), (0, 0.10, 3, 10)], seed=5, matrix=1)
+
+
+def c3_batch(count=64, sizes=(32, 48, 64, 96, 128)):
+    rng = np.random.default_rng(1000)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        g = int(rng.integers(20, 201))
+        r = np.random.default_rng(1000 + i)
+        E = r.uniform(50e9, 300e9, size=g)
+        nu = r.uniform(0.2, 0.35, size=g)
+        mat = np.zeros((g, 7))
+        mat[:, 0], mat[:, 1], mat[:, 2], mat[:, 4] = E, nu, np.inf, 1
+        out.append((n, voronoi(n, g, 1000 + i), mat))
+    return out
