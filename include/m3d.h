@@ -103,6 +103,15 @@ int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0,
  * "chunk" (iterations per graph replay). */
 int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 
+/* Kernel timing for measurement tools (no reference counterpart).  Times
+ * `reps` standalone launches of each per-iteration kernel of the given
+ * precision over the batch of the last run/homogenization, with HIP events
+ * on the handle's stream.  ms6 = average ms per launch of {z pass with
+ * update, y pass forward, x pass with Green step, y pass inverse, z pass of
+ * the check FFT, x pass of the check FFT}; geom9 (may be NULL) =
+ * {Nx, Ny, Nz, Nz/2+1, C, solves, TY, TKy, TKx}.  Overwrites the fields. */
+int m3d_elastic_profile(m3d_elastic *h, int precision, int reps, double *ms6, int *geom9);
+
 const char *m3d_elastic_last_error(m3d_elastic *h);
 
 /* ---------------------------------------------------------------- thermal */

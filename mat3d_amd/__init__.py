@@ -34,7 +34,7 @@ EXPORTS = [
     "m3d_version",
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
-    "m3d_elastic_set_option", "m3d_elastic_last_error",
+    "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
@@ -83,6 +83,7 @@ def lib():
             "m3d_elastic_info": (ip, [vp, dp, ipp, dp, dp]),
             "m3d_elastic_set_option": (ip, [vp, cp, ct.c_double]),
             "m3d_elastic_last_error": (cp, [vp]),
+            "m3d_elastic_profile": (ip, [vp, ip, ip, dp, ipp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -192,6 +193,16 @@ class FFT_Analysis3D:
         sig = np.empty((self.xyz, 6))
         self._chk(lib().m3d_elastic_get_fields(self._h, int(lc), _dp(eps), _dp(sig)))
         return eps, sig
+
+    def profile_kernels(self, precision="fp32_polish", reps=10):
+        """Average ms per launch of the per-iteration kernels (HIP events on the
+        solver's stream) and the geometry used for byte accounting."""
+        ms = np.zeros(6)
+        geom = (ct.c_int * 9)()
+        self._chk(lib().m3d_elastic_profile(self._h, PRECISION[precision], int(reps), _dp(ms), geom))
+        names = ["zpass_update", "ypass_fwd", "xpass_green", "ypass_inv", "zpass_check", "xpass_check"]
+        g = list(geom)
+        return dict(zip(names, ms)), dict(zip(["Nx", "Ny", "Nz", "Nzh", "C", "solves", "TY", "TKy", "TKx"], g))
 
     @property
     def epsilon(self):

@@ -587,6 +587,76 @@ struct Engine {
         HCHK(hipStreamSynchronize(st));
     }
 
+    // Average duration (ms) of standalone launches of the per-iteration
+    // kernels over the current batch: {zpass(update), ypass fwd, xgreen,
+    // ypass inv, zpass(check), xcheck}.  Uses the handle's stream; the field
+    // state is overwritten (benchmark use only).
+    template <typename T, bool GEN>
+    void profile_g(int reps, double* ms)
+    {
+        Geo g = sizeof(T) == 4 ? g32 : g64;
+        g.nsolve = nsolve;
+        Ptr<T> p = ptrs<T>(g);
+        std::vector<Ctl> hctl(nsolve), save(nsolve);
+        HCHK(hipMemcpy(save.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost));
+        hctl = save;
+        for (auto& c : hctl) {
+            c.done = 0;
+            c.need_cur = 1;
+            c.need_prev = 1;
+        }
+        HCHK(hipMemcpy(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice));
+        const int ns = g.nsolve;
+        const size_t zsm = (size_t)g.TY * C * g.lsz * sizeof(cx<T>);
+        const size_t ysm = (size_t)g.Ny * g.TKy * C * sizeof(cx<T>);
+        const size_t xsm = (size_t)g.Nx * g.TKx * C * sizeof(cx<T>);
+        const dim3 gz(ns * g.nbz), gy(ns * g.Nx * g.nky), gx(ns * g.nbx);
+        for (int k = 0; k < 6; ++k) {
+            auto launch = [&]() {
+                switch (k) {
+                case 0: hipLaunchKernelGGL((k_zpass<T, PH, MODE_IUF, GEN>), gz, dim3(ZTHREADS), zsm, st, g, p); break;
+                case 1: hipLaunchKernelGGL((k_ypass<T, C, -1, BUF_MAIN, GEN>), gy, dim3(XYTHREADS), ysm, st, g, p); break;
+                case 2: hipLaunchKernelGGL((k_xgreen<T, PH, GEN>), gx, dim3(XYTHREADS), xsm, st, g, p, prm<T>()); break;
+                case 3: hipLaunchKernelGGL((k_ypass<T, C, 1, BUF_MAIN, GEN>), gy, dim3(XYTHREADS), ysm, st, g, p); break;
+                case 4: hipLaunchKernelGGL((k_zpass<T, PH, MODE_CHK_CUR, GEN>), gz, dim3(ZTHREADS), zsm, st, g, p); break;
+                default: hipLaunchKernelGGL((k_xcheck<T, PH, BUF_CHK_CUR, GEN>), gx, dim3(XYTHREADS), xsm, st, g, p); break;
+                }
+            };
+            launch(); // warm
+            HCHK(hipEventRecord(ev0, st));
+            for (int r = 0; r < reps; ++r) launch();
+            HCHK(hipEventRecord(ev1, st));
+            HCHK(hipEventSynchronize(ev1));
+            float t = 0;
+            HCHK(hipEventElapsedTime(&t, ev0, ev1));
+            ms[k] = (double)t / reps;
+        }
+        HCHK(hipGetLastError());
+        HCHK(hipMemcpy(ctl, save.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice));
+    }
+
+    void profile(int prec, int reps, double* ms)
+    {
+        if (nsolve < 1) throw ArgError{M3D_ERR_STATE, "profile: run or homogenize first"};
+        if (prec == M3D_FP32_POLISH && precision != M3D_FP32_POLISH)
+            throw ArgError{M3D_ERR_ARG, "profile: FP32 kernels need an FP32+polish handle"};
+        if (prec == M3D_FP32_POLISH) {
+            if (needs_gen(g32)) profile_g<float, true>(reps, ms);
+            else profile_g<float, false>(reps, ms);
+        } else {
+            if (needs_gen(g64)) profile_g<double, true>(reps, ms);
+            else profile_g<double, false>(reps, ms);
+        }
+    }
+
+    // geometry summary for byte accounting: {Nx,Ny,Nz,Nzh,C,nsolve,TY,TKy,TKx}
+    void geometry(int prec, int* out)
+    {
+        const Geo& g = prec == M3D_FP32_POLISH ? g32 : g64;
+        int v[9] = {g.Nx, g.Ny, g.Nz, g.Nzh, g.C, nsolve, g.TY, g.TKy, g.TKx};
+        std::memcpy(out, v, sizeof v);
+    }
+
     // upload phase ids (validated, < 256 phases) and coefficient tables
     void upload_material(const std::vector<uint8_t>& ids, const std::vector<double>& coef, const std::vector<double>& prmv)
     {
@@ -903,6 +973,17 @@ int m3d_elastic_get_fields(m3d_elastic* h, int lc, double* eps, double* sig)
         // u is epsilon in the strain base and sigma in the stress base
         if (h->strain_base) h->eng->fields(lc, eps, sig);
         else h->eng->fields(lc, sig, eps);
+        return M3D_OK;
+    });
+}
+
+int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, int* geom9)
+{
+    return guarded(h, [&]() {
+        if (!ms6 || reps < 1) throw ArgError{M3D_ERR_ARG, "profile: bad arguments"};
+        h->eng->profile(precision, reps, ms6);
+        if (geom9) h->eng->geometry(precision, geom9);
+        h->ran = false; // the fields were overwritten
         return M3D_OK;
     });
 }

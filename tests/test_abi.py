@@ -1,0 +1,40 @@
+"""CPU checks of the C-ABI library: it loads and exports every entry point
+include/m3d.h declares (no compute calls: there is no GPU here)."""
+import ctypes as ct
+import os
+import re
+
+import mat3d_amd as m
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "m3d.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(m3d_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_python_export_list():
+    assert sorted(m.EXPORTS) == declared_symbols()
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = ct.CDLL(m.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert m.lib().m3d_version().startswith(b"mat3d_amd")
+
+
+def test_null_handle_is_an_error_not_a_crash():
+    L = m.lib()
+    assert L.m3d_elastic_destroy(None) == -1
+    assert L.m3d_thermal_destroy(None) == -1
+    assert L.m3d_elastic_set_load(None, None, None) == -1
+    assert L.m3d_elastic_last_error(None) == b"null handle"
+    out = ct.c_void_p()
+    # argument validation happens before any device call
+    assert L.m3d_elastic_create(1, 8, 8, b"e", 0, ct.byref(out)) == -1
+    assert L.m3d_elastic_create(8, 8, 8, b"xx", 0, ct.byref(out)) == -1
+    assert L.m3d_elastic_create(8, 8, 8, b"e", 7, ct.byref(out)) == -1
+    assert L.m3d_thermal_create(8, 8, 0, 0, ct.byref(out)) == -1
