@@ -98,9 +98,10 @@ int m3d_elastic_get_fields(m3d_elastic *h, int lc, double *eps, double *sig);
  * strain_base: 1 strain-based, 0 stress-based. */
 int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0, double *mu0);
 
-/* Tuning knobs (no reference counterpart): key "polish_factor" (FP32 phase
- * stops at tol*factor, default 10), "cap" (iteration cap, default 2000),
- * "chunk" (iterations per graph replay). */
+/* Tuning knobs (no reference counterpart): key "polish_factor" (the FP32
+ * phase stops at max(tol*factor, fp32_floor), default 10), "fp32_floor"
+ * (default 1e-6), "cap" (iteration cap, default 2000 elastic / 500 thermal),
+ * "chunk" (iterations per graph replay, 0 = automatic). */
 int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 
 /* Kernel timing for measurement tools (no reference counterpart).  Times

@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "m3d_host.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("m3d_elastic.hip", "m3d_thermal.hip")]
 LIB = os.path.join(HERE, "libmat3d.so")
 
 HIPCC_FLAGS = [
@@ -23,16 +23,35 @@ HIPCC_FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
+    """Compile the two ABI units in parallel, then link libmat3d.so."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", LIB + ".tmp", SRC]
+    objs, procs = [], []
+    for src in SRCS:
+        obj = os.path.join(HERE, "csrc", os.path.basename(src).replace(".hip", ".o"))
+        cmd = [hipcc, *[f for f in HIPCC_FLAGS if f != "-shared"], "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+        objs.append(obj)
+    failed = False
+    for pr in procs:
+        out, _ = pr.communicate()
+        msgs = [l for l in out.splitlines() if "packed-fp32-ops" not in l]
+        if pr.returncode != 0:
+            failed = True
+            sys.stderr.write("\n".join(msgs) + "\n")
+    if failed:
+        raise RuntimeError("hipcc failed building libmat3d.so")
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
-    msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l]
     if r.returncode != 0:
-        sys.stderr.write("\n".join(msgs) + "\n")
-        raise RuntimeError("hipcc failed building libmat3d.so")
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link of libmat3d.so failed")
     os.replace(LIB + ".tmp", LIB)
+    for o in objs:
+        os.remove(o)
     return LIB
 
 

@@ -1,0 +1,783 @@
+#pragma once
+// m3d_engine.hpp -- host runtime of libmat3d.so (gfx950), shared by the ABI units.
+//
+// Owns the device state of one solver handle, builds the FFT plans and tile
+// shapes, drives the iteration as captured hipGraphs of a chunk of
+// iterations and assembles the homogenized tensors in FP64 on the host.
+// Control decisions (stop rule, equilibrium-check scheduling) are taken on
+// the device; the host only polls the per-solve done flags between graph
+// replays.
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/m3d.h"
+#include "m3d_kernels.hpp"
+#include "m3d_fast.hpp"
+
+using namespace m3d;
+
+namespace {
+
+constexpr size_t LDS_MAX = 65536;
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HCHK(x)                                                                                                        \
+    do {                                                                                                               \
+        hipError_t _e = (x);                                                                                           \
+        if (_e != hipSuccess) throw HipError{_e, #x};                                                                  \
+    } while (0)
+
+struct ArgError {
+    int code;
+    std::string msg;
+};
+
+// ------------------------------------------------------------ FFT planning
+bool make_axis(int L, int table_len, FftAxis& a)
+{
+    std::memset(&a, 0, sizeof a);
+    a.L = L;
+    a.tws = table_len / L;
+    int m = L, e2 = 0;
+    while (m % 2 == 0) { m /= 2; ++e2; }
+    std::vector<int> r;
+    // powers of two: radix 8 where possible, 4*4 instead of 8*2
+    while (e2 >= 3) {
+        if (e2 == 4) { r.push_back(4); r.push_back(4); e2 = 0; break; }
+        r.push_back(8);
+        e2 -= 3;
+    }
+    if (e2 == 2) r.push_back(4);
+    if (e2 == 1) r.push_back(2);
+    for (int p : {3, 5, 7, 11, 13})
+        while (m % p == 0) { r.push_back(p); m /= p; }
+    if (m != 1 || (int)r.size() > 16) return false;
+    a.np = (int)r.size();
+    a.rad = 0;
+    for (int i = 0; i < a.np; ++i) a.rad |= (unsigned long long)r[i] << (4 * i);
+    return true;
+}
+
+template <typename T>
+std::vector<cx<T>> twiddles(int n)
+{
+    std::vector<cx<T>> t(n);
+    for (int m = 0; m < n; ++m) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        t[m] = cx<T>{(T)cosl(a), (T)(-sinl(a))};
+    }
+    return t;
+}
+
+// tile shapes for precision of complex size csz
+void plan_tiles(Geo& g, size_t csz)
+{
+    const int C = g.C;
+    const int Lz = g.zeven ? g.Nz / 2 : g.Nz;
+    g.lsz = g.zeven ? g.Nzh : g.Nz;
+    g.TY = 0;
+    for (int ty = g.Ny; ty >= 1; --ty) {
+        if (g.Ny % ty) continue;
+        if ((long)ty * C * Lz > (long)EPT * ZTHREADS) continue;
+        if ((size_t)ty * C * g.lsz * csz > LDS_MAX) continue;
+        g.TY = ty;
+        break;
+    }
+    if (!g.TY) throw ArgError{M3D_ERR_UNSUPPORTED, "z line too long for one workgroup"};
+    g.nty = g.Ny / g.TY;
+    auto tk = [&](int L) {
+        long a = (long)EPT * XYTHREADS / ((long)C * L);
+        long b = (long)(LDS_MAX / ((size_t)C * L * csz));
+        long t = std::min(a, b);
+        t = std::min<long>(t, g.Nzh);
+        return (int)t;
+    };
+    g.TKy = tk(g.Ny);
+    g.TKx = tk(g.Nx);
+    if (g.TKy < 1 || g.TKx < 1) throw ArgError{M3D_ERR_UNSUPPORTED, "x/y line too long for one workgroup"};
+    g.nky = (g.Nzh + g.TKy - 1) / g.TKy;
+    g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
+    g.nbz = g.Nx * g.nty;
+    g.nbx = g.Ny * g.nkx;
+}
+
+// line lengths with compile-time specialised kernels (per physics)
+bool fast_xy(int C, int L)
+{
+    if (C == 6) return L == 64 || L == 128 || L == 256 || L == 192 || L == 96 || L == 32 || L == 48;
+    return L == 64 || L == 128 || L == 256 || L == 512;
+}
+bool fast_z(int C, int LZ)
+{
+    if (C == 6) return LZ == 32 || LZ == 64 || LZ == 128 || LZ == 96 || LZ == 48 || LZ == 16 || LZ == 24;
+    return LZ == 32 || LZ == 64 || LZ == 128 || LZ == 256;
+}
+
+template <typename T>
+Geo make_geo(int Nx, int Ny, int Nz, int C)
+{
+    const size_t csz = sizeof(cx<T>);
+    Geo g;
+    std::memset(&g, 0, sizeof g);
+    g.Nx = Nx; g.Ny = Ny; g.Nz = Nz; g.Nzh = Nz / 2 + 1; g.C = C;
+    g.N = (long long)Nx * Ny * Nz;
+    g.H = (long long)Nx * Ny * g.Nzh;
+    g.zeven = (Nz % 2 == 0);
+    g.invN = 1.0 / (double)g.N;
+    if (!make_axis(g.zeven ? Nz / 2 : Nz, Nz, g.az) || !make_axis(Ny, Ny, g.ay) || !make_axis(Nx, Nx, g.ax))
+        throw ArgError{M3D_ERR_UNSUPPORTED, "grid size has a prime factor larger than 13"};
+    plan_tiles(g, csz);
+    const FftAxis* a[3] = {&g.az, &g.ay, &g.ax};
+    for (const FftAxis* x : a)
+        for (int i = 0; i < x->np; ++i) {
+            const int r = x->radix(i);
+            if (r != 2 && r != 3 && r != 4 && r != 8) g.gen = 1;
+        }
+    // compile-time specialised passes where available
+    // the z kernels are instantiated for the tile a cubic grid would use
+    if (g.zeven && fast_z(C, Nz / 2) && fast::zlines<T>(Nz / 2, C, Ny) == fast::zlines<T>(Nz / 2, C, Nz)) {
+        g.fz = 1;
+        g.TY = fast::zlines<T>(Nz / 2, C, Ny);
+        g.nty = Ny / g.TY;
+        g.nbz = Nx * g.nty;
+    }
+    if (fast_xy(C, Ny)) {
+        g.fy = 1;
+        g.TKy = fast::xytile<T>(Ny, C);
+        g.nky = (g.Nzh + g.TKy - 1) / g.TKy;
+    }
+    if (fast_xy(C, Nx)) {
+        g.fx = 1;
+        g.TKx = fast::xytile<T>(Nx, C);
+        g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
+        g.nbx = Ny * g.nkx;
+    }
+    return g;
+}
+
+// ---------------------------------------------------------- small algebra
+bool inv_n(const double* A, double* X, int n)
+{
+    double M[36];
+    for (int i = 0; i < n * n; ++i) { M[i] = A[i]; X[i] = 0; }
+    for (int i = 0; i < n; ++i) X[i + i * n] = 1;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        double best = std::fabs(M[c + c * n]);
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[r + c * n]) > best) { best = std::fabs(M[r + c * n]); p = r; }
+        if (best == 0.0 || !std::isfinite(best)) return false;
+        if (p != c)
+            for (int k = 0; k < n; ++k) { std::swap(M[c + k * n], M[p + k * n]); std::swap(X[c + k * n], X[p + k * n]); }
+        double d = M[c + c * n];
+        for (int k = 0; k < n; ++k) { M[c + k * n] /= d; X[c + k * n] /= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = M[r + c * n];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) { M[r + k * n] -= f * M[c + k * n]; X[r + k * n] -= f * X[c + k * n]; }
+        }
+    }
+    return true;
+}
+
+void mul_n(const double* A, const double* B, double* Cm, int n)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = 0; k < n; ++k) s += A[i + k * n] * B[k + j * n];
+            Cm[i + j * n] = s;
+        }
+}
+
+void iso6(double la, double mu, double* C6)
+{
+    std::memset(C6, 0, sizeof(double) * 36);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C6[i + 6 * j] = la + (i == j ? 2 * mu : 0.0);
+    for (int i = 3; i < 6; ++i) C6[i + 6 * i] = 2 * mu;
+}
+
+// max_min_Array semantics (first index of the extreme), FFT_analysis3D.cpp:5-23
+void max_min(const std::vector<double>& a, double& mx, int& imx, double& mn)
+{
+    mx = a[0]; imx = 0; mn = a[0];
+    for (size_t i = 1; i < a.size(); ++i) {
+        if (a[i] > mx) { mx = a[i]; imx = (int)i; }
+        if (a[i] < mn) mn = a[i];
+    }
+}
+
+// --------------------------------------------------------------- engine
+template <typename PH>
+struct Engine {
+    static constexpr int C = PH::C;
+    int Nx, Ny, Nz;
+    int precision;
+    Geo g32{}, g64{};
+    int cap_solves = 0;
+    long long N = 0, H = 0;
+    // device buffers
+    float* U32[2] = {nullptr, nullptr};
+    double* U64[2] = {nullptr, nullptr};
+    unsigned char* Sbuf = nullptr;
+    unsigned char* Kbuf = nullptr;
+    double* psum = nullptr;
+    double* pchk = nullptr;
+    Ctl* ctl = nullptr;
+    double* mpart = nullptr;
+    int* rve_of = nullptr;
+    uint8_t* ph = nullptr;
+    float* coef32 = nullptr;
+    double* coef64 = nullptr;
+    float* prm32 = nullptr;
+    double* prm64 = nullptr;
+    cx<float>* tw32[3] = {nullptr, nullptr, nullptr};
+    cx<double>* tw64[3] = {nullptr, nullptr, nullptr};
+    double* fbuf = nullptr; // field output staging (2*C*N doubles)
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // host state
+    int nsolve = 0;
+    std::vector<int> final_prec; // per solve: 0 fp32, 1 fp64
+    std::vector<m3d_run_info> info;
+    double polish_factor = 10.0;
+    double fp32_floor = 1e-6;
+    int cap = 2000;
+    int chunk = 0;
+    int mnblk = 256;
+
+    Engine(int nx, int ny, int nz, int prec) : Nx(nx), Ny(ny), Nz(nz), precision(prec)
+    {
+        g32 = make_geo<float>(nx, ny, nz, C);
+        g64 = make_geo<double>(nx, ny, nz, C);
+        N = g64.N;
+        H = g64.H;
+        HCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HCHK(hipEventCreate(&ev0));
+        HCHK(hipEventCreate(&ev1));
+        // twiddle tables: z table has length Nz, y: Ny, x: Nx
+        const int lens[3] = {Nz, Ny, Nx};
+        for (int a = 0; a < 3; ++a) {
+            auto t32 = twiddles<float>(lens[a]);
+            auto t64 = twiddles<double>(lens[a]);
+            HCHK(hipMalloc(&tw32[a], sizeof(cx<float>) * lens[a]));
+            HCHK(hipMalloc(&tw64[a], sizeof(cx<double>) * lens[a]));
+            HCHK(hipMemcpy(tw32[a], t32.data(), sizeof(cx<float>) * lens[a], hipMemcpyHostToDevice));
+            HCHK(hipMemcpy(tw64[a], t64.data(), sizeof(cx<double>) * lens[a], hipMemcpyHostToDevice));
+        }
+        HCHK(hipMalloc(&ph, (size_t)N));
+    }
+
+    void free_solves()
+    {
+        for (int b = 0; b < 2; ++b) {
+            if (U32[b]) hipFree(U32[b]);
+            if (U64[b]) hipFree(U64[b]);
+            U32[b] = nullptr;
+            U64[b] = nullptr;
+        }
+        if (Sbuf) hipFree(Sbuf);
+        if (Kbuf) hipFree(Kbuf);
+        if (psum) hipFree(psum);
+        if (pchk) hipFree(pchk);
+        if (ctl) hipFree(ctl);
+        if (mpart) hipFree(mpart);
+        if (rve_of) hipFree(rve_of);
+        Sbuf = Kbuf = nullptr;
+        psum = pchk = mpart = nullptr;
+        ctl = nullptr;
+        rve_of = nullptr;
+        cap_solves = 0;
+    }
+
+    ~Engine()
+    {
+        if (st) hipStreamSynchronize(st);
+        free_solves();
+        if (ph) hipFree(ph);
+        if (coef32) hipFree(coef32);
+        if (coef64) hipFree(coef64);
+        if (prm32) hipFree(prm32);
+        if (prm64) hipFree(prm64);
+        if (fbuf) hipFree(fbuf);
+        for (int a = 0; a < 3; ++a) {
+            if (tw32[a]) hipFree(tw32[a]);
+            if (tw64[a]) hipFree(tw64[a]);
+        }
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (st) hipStreamDestroy(st);
+    }
+
+    template <typename P>
+    void alloc(P** p, size_t bytes)
+    {
+        hipError_t e = hipMalloc((void**)p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            throw ArgError{M3D_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed"};
+        }
+    }
+
+    void ensure_solves(int n)
+    {
+        if (n <= cap_solves) return;
+        free_solves();
+        const size_t nc = (size_t)N * C;
+        const size_t hc = (size_t)H * C;
+        if (precision == M3D_FP32_POLISH) {
+            alloc(&U32[0], sizeof(float) * nc * n);
+            alloc(&U32[1], sizeof(float) * nc * n);
+        }
+        alloc(&U64[0], sizeof(double) * nc * n);
+        alloc(&U64[1], sizeof(double) * nc * n);
+        alloc(&Sbuf, sizeof(cx<double>) * hc * n);
+        alloc(&Kbuf, sizeof(cx<double>) * hc * n);
+        const int nbz = std::max(g32.nbz, g64.nbz), nbx = std::max(g32.nbx, g64.nbx);
+        alloc(&psum, sizeof(double) * (size_t)nbz * n);
+        alloc(&pchk, sizeof(double) * (size_t)2 * (nbx + 1) * n);
+        alloc(&ctl, sizeof(Ctl) * n);
+        alloc(&mpart, sizeof(double) * (size_t)2 * C * mnblk * n);
+        alloc(&rve_of, sizeof(int) * n);
+        std::vector<int> z(n, 0);
+        HCHK(hipMemcpy(rve_of, z.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        cap_solves = n;
+    }
+
+    template <typename T>
+    Ptr<T> ptrs(const Geo& g)
+    {
+        Ptr<T> p;
+        std::memset(&p, 0, sizeof p);
+        const size_t nc = (size_t)N * C;
+        if constexpr (sizeof(T) == 4) {
+            p.U0 = U32[0];
+            p.U1 = U32[1];
+            p.coef = coef32;
+            p.twz = tw32[0];
+            p.twy = tw32[1];
+            p.twx = tw32[2];
+        } else {
+            p.U0 = U64[0];
+            p.U1 = U64[1];
+            p.coef = coef64;
+            p.twz = tw64[0];
+            p.twy = tw64[1];
+            p.twx = tw64[2];
+        }
+        p.sU = (long long)nc;
+        p.S = reinterpret_cast<cx<T>*>(Sbuf);
+        p.K = reinterpret_cast<cx<T>*>(Kbuf);
+        p.sS = (long long)H * C * (sizeof(cx<double>) / sizeof(cx<T>));
+        p.ph = ph;
+        p.sPh = N;
+        p.rve_of = rve_of;
+        p.sCoef = 0;
+        p.psum = psum;
+        p.pchk = pchk;
+        p.ctl = ctl;
+        (void)g;
+        return p;
+    }
+
+    template <typename T>
+    const T* prm()
+    {
+        if constexpr (sizeof(T) == 4) return prm32;
+        else return prm64;
+    }
+
+    // ---------------------------------------------------------- launches
+    // ------------------------------------------------- per-pass dispatch
+    template <typename T, int MODE, int LZ>
+    bool zf(const Geo& g, const Ptr<T>& p)
+    {
+        if (g.Nz != 2 * LZ) return false;
+        constexpr int TYc = fast::zlines<T>(LZ, C, 2 * LZ);
+        if (g.TY != TYc) throw ArgError{M3D_ERR_UNSUPPORTED, "z tile mismatch"};
+        const size_t sm = ((size_t)(LZ + 1) * TYc * C + 2 * LZ) * sizeof(cx<T>);
+        hipLaunchKernelGGL((fast::kz<T, PH, MODE, LZ, TYc>), dim3(g.nsolve * g.nbz), dim3(fast::ZNT), sm, st, g, p);
+        return true;
+    }
+
+    template <typename T, int MODE>
+    void zpass(const Geo& g, const Ptr<T>& p)
+    {
+        if (g.fz) {
+            bool ok;
+            if constexpr (C == 6)
+                ok = zf<T, MODE, 128>(g, p) || zf<T, MODE, 64>(g, p) || zf<T, MODE, 32>(g, p) || zf<T, MODE, 96>(g, p) ||
+                     zf<T, MODE, 48>(g, p) || zf<T, MODE, 16>(g, p) || zf<T, MODE, 24>(g, p);
+            else
+                ok = zf<T, MODE, 256>(g, p) || zf<T, MODE, 128>(g, p) || zf<T, MODE, 64>(g, p) || zf<T, MODE, 32>(g, p);
+            if (!ok) throw ArgError{M3D_ERR_UNSUPPORTED, "no z kernel"};
+            return;
+        }
+        const size_t zsm = (size_t)g.TY * C * g.lsz * sizeof(cx<T>);
+        const dim3 grid(g.nsolve * g.nbz);
+        if (g.gen) hipLaunchKernelGGL((k_zpass<T, PH, MODE, true>), grid, dim3(ZTHREADS), zsm, st, g, p);
+        else hipLaunchKernelGGL((k_zpass<T, PH, MODE, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
+    }
+
+    // KIND 0: y pass, 1: x pass + Green, 2: x pass of the check FFT
+    template <typename T, int KIND, int DIR, int WHICH, int L>
+    bool xyf(const Geo& g, const Ptr<T>& p)
+    {
+        const int Lax = KIND == 0 ? g.Ny : g.Nx;
+        if (Lax != L) return false;
+        const int TK = KIND == 0 ? g.TKy : g.TKx;
+        constexpr int TKc = fast::xytile<T>(L, C);
+        const int nkt = (g.Nzh + TK - 1) / TK;
+        const int outer = KIND == 0 ? g.Nx : g.Ny;
+        const dim3 grid(g.nsolve * outer * nkt);
+        if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
+        const size_t sm = ((size_t)L * TKc * C + L) * sizeof(cx<T>);
+        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>), grid, dim3(fast::XYNT), sm, st, g, p, prm<T>());
+        return true;
+    }
+
+    template <typename T, int KIND, int DIR, int WHICH>
+    bool xyfast(const Geo& g, const Ptr<T>& p)
+    {
+        if constexpr (C == 6)
+            return xyf<T, KIND, DIR, WHICH, 256>(g, p) || xyf<T, KIND, DIR, WHICH, 128>(g, p) ||
+                   xyf<T, KIND, DIR, WHICH, 64>(g, p) || xyf<T, KIND, DIR, WHICH, 192>(g, p) ||
+                   xyf<T, KIND, DIR, WHICH, 96>(g, p) || xyf<T, KIND, DIR, WHICH, 32>(g, p) ||
+                   xyf<T, KIND, DIR, WHICH, 48>(g, p);
+        else
+            return xyf<T, KIND, DIR, WHICH, 512>(g, p) || xyf<T, KIND, DIR, WHICH, 256>(g, p) ||
+                   xyf<T, KIND, DIR, WHICH, 128>(g, p) || xyf<T, KIND, DIR, WHICH, 64>(g, p);
+    }
+
+    template <typename T, int DIR, int WHICH>
+    void ypass(const Geo& g, const Ptr<T>& p)
+    {
+        if (g.fy) {
+            if (!xyfast<T, 0, DIR, WHICH>(g, p)) throw ArgError{M3D_ERR_UNSUPPORTED, "no y kernel"};
+            return;
+        }
+        const size_t ysm = (size_t)g.Ny * g.TKy * C * sizeof(cx<T>);
+        const dim3 grid(g.nsolve * g.Nx * g.nky);
+        if (g.gen) hipLaunchKernelGGL((k_ypass<T, C, DIR, WHICH, true>), grid, dim3(XYTHREADS), ysm, st, g, p);
+        else hipLaunchKernelGGL((k_ypass<T, C, DIR, WHICH, false>), grid, dim3(XYTHREADS), ysm, st, g, p);
+    }
+
+    template <typename T>
+    void xgreen(const Geo& g, const Ptr<T>& p)
+    {
+        if (g.fx) {
+            if (!xyfast<T, 1, -1, BUF_MAIN>(g, p)) throw ArgError{M3D_ERR_UNSUPPORTED, "no x kernel"};
+            return;
+        }
+        const size_t xsm = (size_t)g.Nx * g.TKx * C * sizeof(cx<T>);
+        const dim3 grid(g.nsolve * g.nbx);
+        if (g.gen) hipLaunchKernelGGL((k_xgreen<T, PH, true>), grid, dim3(XYTHREADS), xsm, st, g, p, prm<T>());
+        else hipLaunchKernelGGL((k_xgreen<T, PH, false>), grid, dim3(XYTHREADS), xsm, st, g, p, prm<T>());
+    }
+
+    template <typename T, int WHICH>
+    void xcheck(const Geo& g, const Ptr<T>& p)
+    {
+        if (g.fx) {
+            if (!xyfast<T, 2, -1, WHICH>(g, p)) throw ArgError{M3D_ERR_UNSUPPORTED, "no x check kernel"};
+            return;
+        }
+        const size_t xsm = (size_t)g.Nx * g.TKx * C * sizeof(cx<T>);
+        const dim3 grid(g.nsolve * g.nbx);
+        if (g.gen) hipLaunchKernelGGL((k_xcheck<T, PH, WHICH, true>), grid, dim3(XYTHREADS), xsm, st, g, p);
+        else hipLaunchKernelGGL((k_xcheck<T, PH, WHICH, false>), grid, dim3(XYTHREADS), xsm, st, g, p);
+    }
+
+    template <typename T>
+    void launch_start(const Geo& g, const Ptr<T>& p)
+    {
+        zpass<T, MODE_FWD>(g, p);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
+        ypass<T, -1, BUF_MAIN>(g, p);
+        xgreen<T>(g, p);
+        ypass<T, 1, BUF_MAIN>(g, p);
+    }
+
+    template <typename T>
+    void launch_iter(const Geo& g, const Ptr<T>& p)
+    {
+        zpass<T, MODE_IUF>(g, p);
+        hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
+        zpass<T, MODE_CHK_CUR>(g, p);
+        ypass<T, -1, BUF_CHK_CUR>(g, p);
+        xcheck<T, BUF_CHK_CUR>(g, p);
+        zpass<T, MODE_CHK_PREV>(g, p);
+        ypass<T, -1, BUF_CHK_PREV>(g, p);
+        xcheck<T, BUF_CHK_PREV>(g, p);
+        hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.pchk);
+        ypass<T, -1, BUF_MAIN>(g, p);
+        xgreen<T>(g, p);
+        ypass<T, 1, BUF_MAIN>(g, p);
+    }
+
+    int chunk_for(const Geo& g) const
+    {
+        if (chunk > 0) return chunk;
+        const double work = (double)g.N * g.nsolve;
+        if (work >= 3.0e7) return 2;
+        if (work >= 4.0e6) return 4;
+        return 8;
+    }
+
+    // run the iteration in precision T until every solve of the batch is done
+    template <typename T>
+    void run_phase(Geo g, std::vector<Ctl>& hctl)
+    {
+        g.nsolve = nsolve;
+        Ptr<T> p = ptrs<T>(g);
+        launch_start<T>(g, p);
+        HCHK(hipGetLastError());
+        const int ck = chunk_for(g);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < ck; ++i) launch_iter<T>(g, p);
+        hipError_t ce = hipStreamEndCapture(st, &graph);
+        if (ce != hipSuccess) throw HipError{ce, "hipStreamEndCapture"};
+        ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            hipGraphDestroy(graph);
+            throw HipError{ce, "hipGraphInstantiate"};
+        }
+        try {
+            for (int guard = 0; guard < 100000; ++guard) {
+                HCHK(hipGraphLaunch(exec, st));
+                HCHK(hipMemcpyAsync(hctl.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost, st));
+                HCHK(hipStreamSynchronize(st));
+                bool all = true;
+                for (int s = 0; s < nsolve; ++s) all = all && hctl[s].done;
+                if (all) break;
+            }
+        } catch (...) {
+            hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+            throw;
+        }
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+    }
+
+    // loads: nsolve x 6 (only the first C entries used); tol per phase
+    void solve(const std::vector<std::array<double, 6>>& loads, double tol_f, double tol_eq, const std::vector<double>& norm)
+    {
+        nsolve = (int)loads.size();
+        ensure_solves(nsolve);
+        std::vector<Ctl> hctl(nsolve);
+        for (int s = 0; s < nsolve; ++s) {
+            Ctl& c = hctl[s];
+            std::memset(&c, 0, sizeof c);
+            c.eq_it = -1;
+            c.cap = cap;
+            c.norm = norm[s];
+            for (int k = 0; k < 6; ++k) c.E[k] = loads[s][k];
+            // the FP32 phase stops at polish_factor x tol, but never below what
+            // FP32 fields resolve (fp32_floor); the FP64 polish does the rest
+            const bool two = (precision == M3D_FP32_POLISH);
+            c.tol_f = two ? std::max(tol_f * polish_factor, fp32_floor) : tol_f;
+            c.tol_eq = two ? std::max(tol_eq * polish_factor, fp32_floor) : tol_eq;
+        }
+        HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
+        final_prec.assign(nsolve, 1);
+        info.assign(nsolve, m3d_run_info{});
+        HCHK(hipEventRecord(ev0, st));
+        Geo gf = g32, gd = g64;
+        gf.nsolve = gd.nsolve = nsolve;
+        if (precision == M3D_FP32_POLISH) {
+            Ptr<float> pf = ptrs<float>(gf);
+            hipLaunchKernelGGL((k_fill_load<float, C>), dim3(1024), dim3(256), 0, st, gf, pf, 0);
+            run_phase<float>(gf, hctl);
+            std::vector<int> it32(nsolve);
+            for (int s = 0; s < nsolve; ++s) {
+                it32[s] = hctl[s].it;
+                info[s].iters_fp32 = hctl[s].it;
+                if (hctl[s].done == 1) {
+                    hctl[s].done = 0;
+                    hctl[s].tol_f = tol_f;
+                    hctl[s].tol_eq = tol_eq;
+                } else {
+                    final_prec[s] = 0; // iteration cap reached in FP32: keep that state
+                }
+            }
+            hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, U32[0], U32[1], (long long)N * C, U64[0],
+                               U64[1], (long long)N * C, ctl);
+            HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
+            bool any = false;
+            for (int s = 0; s < nsolve; ++s) any = any || (hctl[s].done == 0);
+            if (any) run_phase<double>(gd, hctl);
+            for (int s = 0; s < nsolve; ++s) info[s].iters_fp64 = hctl[s].it - it32[s];
+        } else {
+            Ptr<double> pd = ptrs<double>(gd);
+            hipLaunchKernelGGL((k_fill_load<double, C>), dim3(1024), dim3(256), 0, st, gd, pd, 0);
+            run_phase<double>(gd, hctl);
+            for (int s = 0; s < nsolve; ++s) info[s].iters_fp64 = hctl[s].it;
+        }
+        HCHK(hipEventRecord(ev1, st));
+        HCHK(hipStreamSynchronize(st));
+        float ms = 0;
+        HCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        for (int s = 0; s < nsolve; ++s) {
+            info[s].iterations = hctl[s].it;
+            info[s].status = hctl[s].done;
+            info[s].eqVar = hctl[s].eqvar;
+            info[s].diff_sum = hctl[s].dsum;
+            info[s].diff_eq = hctl[s].deq;
+            info[s].time_ms = ms;
+        }
+    }
+
+    // means of u and v per solve: out[s][0..C) = <u>, out[s][C..2C) = <v>
+    std::vector<double> means()
+    {
+        std::vector<double> out((size_t)nsolve * 2 * C, 0.0);
+        std::vector<double> part((size_t)nsolve * 2 * C * mnblk);
+        for (int pr = 0; pr < 2; ++pr) {
+            bool any = false;
+            for (int s = 0; s < nsolve; ++s) any = any || (final_prec[s] == pr);
+            if (!any) continue;
+            if (pr == 0) {
+                Geo g = g32;
+                g.nsolve = nsolve;
+                hipLaunchKernelGGL((k_means<float, PH>), dim3(mnblk, nsolve), dim3(256), 0, st, g, ptrs<float>(g), mpart, mnblk);
+            } else {
+                Geo g = g64;
+                g.nsolve = nsolve;
+                hipLaunchKernelGGL((k_means<double, PH>), dim3(mnblk, nsolve), dim3(256), 0, st, g, ptrs<double>(g), mpart, mnblk);
+            }
+            HCHK(hipGetLastError());
+            HCHK(hipMemcpyAsync(part.data(), mpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            for (int s = 0; s < nsolve; ++s) {
+                if (final_prec[s] != pr) continue;
+                for (int q = 0; q < 2 * C; ++q) {
+                    double a = 0;
+                    for (int b = 0; b < mnblk; ++b) a += part[((size_t)s * 2 * C + q) * mnblk + b];
+                    out[(size_t)s * 2 * C + q] = a / (double)N;
+                }
+            }
+        }
+        return out;
+    }
+
+    void fields(int s, double* uo, double* vo)
+    {
+        if (s < 0 || s >= nsolve) throw ArgError{M3D_ERR_ARG, "load case index out of range"};
+        if (!fbuf) alloc(&fbuf, sizeof(double) * 2 * C * (size_t)N);
+        double* du = fbuf;
+        double* dv = fbuf + (size_t)C * N;
+        if (final_prec[s] == 0) {
+            Geo g = g32;
+            g.nsolve = nsolve;
+            hipLaunchKernelGGL((k_fields<float, PH>), dim3(2048), dim3(256), 0, st, g, ptrs<float>(g), s, du, dv);
+        } else {
+            Geo g = g64;
+            g.nsolve = nsolve;
+            hipLaunchKernelGGL((k_fields<double, PH>), dim3(2048), dim3(256), 0, st, g, ptrs<double>(g), s, du, dv);
+        }
+        HCHK(hipGetLastError());
+        if (uo) HCHK(hipMemcpyAsync(uo, du, sizeof(double) * C * (size_t)N, hipMemcpyDeviceToHost, st));
+        if (vo) HCHK(hipMemcpyAsync(vo, dv, sizeof(double) * C * (size_t)N, hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+    }
+
+    // Average duration (ms) of standalone launches of the per-iteration
+    // kernels over the current batch: {zpass(update), ypass fwd, xgreen,
+    // ypass inv, zpass(check), xcheck}.  Uses the handle's stream; the field
+    // state is overwritten (benchmark use only).
+    template <typename T>
+    void profile_t(int reps, double* ms)
+    {
+        Geo g = sizeof(T) == 4 ? g32 : g64;
+        g.nsolve = nsolve;
+        Ptr<T> p = ptrs<T>(g);
+        std::vector<Ctl> hctl(nsolve), save(nsolve);
+        HCHK(hipMemcpy(save.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost));
+        hctl = save;
+        for (auto& c : hctl) {
+            c.done = 0;
+            c.need_cur = 1;
+            c.need_prev = 1;
+        }
+        HCHK(hipMemcpy(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice));
+        for (int k = 0; k < 6; ++k) {
+            auto launch = [&]() {
+                switch (k) {
+                case 0: zpass<T, MODE_IUF>(g, p); break;
+                case 1: ypass<T, -1, BUF_MAIN>(g, p); break;
+                case 2: xgreen<T>(g, p); break;
+                case 3: ypass<T, 1, BUF_MAIN>(g, p); break;
+                case 4: zpass<T, MODE_CHK_CUR>(g, p); break;
+                default: xcheck<T, BUF_CHK_CUR>(g, p); break;
+                }
+            };
+            launch(); // warm
+            HCHK(hipEventRecord(ev0, st));
+            for (int r = 0; r < reps; ++r) launch();
+            HCHK(hipEventRecord(ev1, st));
+            HCHK(hipEventSynchronize(ev1));
+            float t = 0;
+            HCHK(hipEventElapsedTime(&t, ev0, ev1));
+            ms[k] = (double)t / reps;
+        }
+        HCHK(hipGetLastError());
+        HCHK(hipMemcpy(ctl, save.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice));
+    }
+
+    void profile(int prec, int reps, double* ms)
+    {
+        if (nsolve < 1) throw ArgError{M3D_ERR_STATE, "profile: run or homogenize first"};
+        if (prec == M3D_FP32_POLISH && precision != M3D_FP32_POLISH)
+            throw ArgError{M3D_ERR_ARG, "profile: FP32 kernels need an FP32+polish handle"};
+        if (prec == M3D_FP32_POLISH) profile_t<float>(reps, ms);
+        else profile_t<double>(reps, ms);
+    }
+
+    // geometry summary for byte accounting: {Nx,Ny,Nz,Nzh,C,nsolve,TY,TKy,TKx}
+    void geometry(int prec, int* out)
+    {
+        const Geo& g = prec == M3D_FP32_POLISH ? g32 : g64;
+        int v[9] = {g.Nx, g.Ny, g.Nz, g.Nzh, g.C, nsolve, g.TY, g.TKy, g.TKx};
+        std::memcpy(out, v, sizeof v);
+    }
+
+    // upload phase ids (validated, < 256 phases) and coefficient tables
+    void upload_material(const std::vector<uint8_t>& ids, const std::vector<double>& coef, const std::vector<double>& prmv)
+    {
+        HCHK(hipMemcpy(ph, ids.data(), (size_t)N, hipMemcpyHostToDevice));
+        std::vector<float> c32(coef.begin(), coef.end());
+        std::vector<float> p32(prmv.begin(), prmv.end());
+        if (coef32) hipFree(coef32);
+        if (coef64) hipFree(coef64);
+        if (prm32) hipFree(prm32);
+        if (prm64) hipFree(prm64);
+        coef32 = nullptr; coef64 = nullptr; prm32 = nullptr; prm64 = nullptr;
+        alloc(&coef32, sizeof(float) * c32.size());
+        alloc(&coef64, sizeof(double) * coef.size());
+        alloc(&prm32, sizeof(float) * p32.size());
+        alloc(&prm64, sizeof(double) * prmv.size());
+        HCHK(hipMemcpy(coef32, c32.data(), sizeof(float) * c32.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(coef64, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(prm32, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(prm64, prmv.data(), sizeof(double) * prmv.size(), hipMemcpyHostToDevice));
+    }
+};
+
+} // namespace
+

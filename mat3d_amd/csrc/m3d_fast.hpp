@@ -1,0 +1,552 @@
+// m3d_fast.hpp -- compile-time specialised passes for the common grid sizes.
+//
+// Same dataflow and layouts as m3d_kernels.hpp, but the line length, the
+// tile shape and the workgroup size are template constants, so every index
+// computation of the Stockham passes folds to shifts and constant
+// multiplies, every loop unrolls and the per-thread register footprint is
+// known.  Twiddle tables are staged in LDS once per workgroup.  Lines are
+// interleaved in LDS (element e of line l at e*W + l) so a wavefront touches
+// consecutive LDS words in every pass.
+//
+// The z pass works on voxel pairs: the half-length complex FFT of a real
+// z-line packs samples (2m, 2m+1) into complex slot m, so the c2r output of
+// slot m of the C component lines *is* the C-vector of voxels 2m and 2m+1.
+// A thread owning slot m reads the 2C unknowns of its voxel pair with vector
+// loads, applies update / dual / polarisation, and writes the 2C polarisation
+// values back into the same slots for the r2c.
+#pragma once
+#include <type_traits>
+#include "m3d_kernels.hpp"
+
+namespace m3d {
+namespace fast {
+
+// ---------------------------------------------------------------- planning
+constexpr int n_twos(int L) { return (L % 2 == 0) ? 1 + n_twos(L / 2) : 0; }
+constexpr int odd_part(int L) { return (L % 2 == 0) ? odd_part(L / 2) : L; }
+constexpr int n_threes(int L) { return (L % 3 == 0) ? 1 + n_threes(L / 3) : 0; }
+// radix sequence: 8s, then 4*4 instead of 8*2, a 4 or a 2, then 3s (same
+// policy as the runtime planner)
+constexpr int n8(int e) { return e >= 3 ? ((e == 4) ? 0 : 1 + n8(e - 3)) : 0; }
+constexpr int rest2(int e) { return e >= 3 ? ((e == 4) ? 4 : rest2(e - 3)) : e; }
+constexpr int npass(int L)
+{
+    int e = n_twos(L), r = rest2(e);
+    int p = n8(e) + (r == 4 ? 2 : (r == 2 ? 1 : (r == 1 ? 1 : 0)));
+    return p + n_threes(L);
+}
+constexpr int radix(int L, int p)
+{
+    int e = n_twos(L), a = n8(e), r = rest2(e);
+    if (p < a) return 8;
+    p -= a;
+    if (r == 4) { if (p < 2) return 4; p -= 2; }
+    else if (r == 2) { if (p < 1) return 4; p -= 1; }
+    else if (r == 1) { if (p < 1) return 2; p -= 1; }
+    return 3;
+}
+constexpr bool supported(int L) { return odd_part(L) == 1 || odd_part(L) == 3 || odd_part(L) == 9; }
+
+// tile shapes (shared with the host planner)
+template <typename T>
+constexpr int zlines(int LZ, int C, int Ny)
+{
+    // TY: largest power of two with TY*C*LZ <= budget that divides Ny
+    int budget = sizeof(T) == 4 ? 3072 : 1536;
+    int ty = 1;
+    while (ty < 16 && ty * 2 * C * LZ <= budget && Ny % (ty * 2) == 0) ty *= 2;
+    return ty;
+}
+template <typename T>
+constexpr int xytile(int L, int C)
+{
+    int budget = sizeof(T) == 4 ? 6144 : 3072;
+    int tk = budget / (C * L);
+    return tk < 1 ? 1 : tk;
+}
+constexpr int ZNT = 256;
+constexpr int XYNT = 256;
+
+// XCD-aware block order: the dispatcher hands block b to XCD b % 8; give
+// each XCD a contiguous run of logical tiles so tiles sharing cache lines
+// run under one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int per = nb >> 3;
+    if (b < (per << 3)) return (b & 7) * per + (b >> 3);
+    return b;
+}
+
+// --------------------------------------------------------------- passes
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w)
+{
+    if (DIR > 0) w.im = -w.im;
+    return cmul(a, w);
+}
+
+// one Stockham pass over W interleaved lines of length L (in place)
+template <typename T, int L, int W, int NT, int R, int Ns, int DIR, int TWS>
+__device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
+{
+    constexpr int nR = L / R;
+    constexpr int items = W * nR;
+    constexpr int IPT = (items + NT - 1) / NT;
+    constexpr int tstep = (L / (Ns * R)) * TWS;
+    const int tid = threadIdx.x;
+    cx<T> v[IPT][R];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+        const int it = tid + i * NT;
+        if (items % NT == 0 || it < items) {
+            const int l = it % W, j = it / W, k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> a = s[(j + r * nR) * W + l];
+                if (Ns > 1 && r > 0) a = twmul<DIR>(a, tw[k * r * tstep]);
+                v[i][r] = a;
+            }
+            butterfly<T, R, DIR>(v[i], tw, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+        const int it = tid + i * NT;
+        if (items % NT == 0 || it < items) {
+            const int l = it % W, j = it / W, k = j % Ns;
+            const int ob = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[(ob + r * Ns) * W + l] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int W, int NT, int DIR, int TWS, int P = 0, int Ns = 1>
+__device__ __forceinline__ void fft(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
+{
+    if constexpr (P < npass(L)) {
+        constexpr int R = radix(L, P);
+        pass<T, L, W, NT, R, Ns, DIR, TWS>(s, tw);
+        fft<T, L, W, NT, DIR, TWS, P + 1, Ns * R>(s, tw);
+    }
+}
+
+// vector load/store of n contiguous values (16-byte chunks when aligned, else 8)
+template <typename T, int n>
+__device__ __forceinline__ void vload(const T* __restrict__ g, T* r)
+{
+    if constexpr ((n * sizeof(T)) % 16 == 0) {
+#pragma unroll
+        for (int i = 0; i < (int)(n * sizeof(T) / 16); ++i) {
+            const float4 q = reinterpret_cast<const float4*>(g)[i];
+            reinterpret_cast<float4*>(r)[i] = q;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < (int)(n * sizeof(T) / 8); ++i) {
+            const float2 q = reinterpret_cast<const float2*>(g)[i];
+            reinterpret_cast<float2*>(r)[i] = q;
+        }
+    }
+}
+template <typename T, int n>
+__device__ __forceinline__ void vstore(T* __restrict__ g, const T* r)
+{
+    if constexpr ((n * sizeof(T)) % 16 == 0) {
+#pragma unroll
+        for (int i = 0; i < (int)(n * sizeof(T) / 16); ++i)
+            reinterpret_cast<float4*>(g)[i] = reinterpret_cast<const float4*>(r)[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < (int)(n * sizeof(T) / 8); ++i)
+            reinterpret_cast<float2*>(g)[i] = reinterpret_cast<const float2*>(r)[i];
+    }
+}
+
+// ---------------------------------------------------------- tile movers
+// Rows of W contiguous complex values in global memory (row r at
+// base + r*stride) <-> LDS rows sl[r*W + j].  All of a thread's loads are
+// issued before any LDS store so the workgroup keeps ROWS*W*sizeof values
+// in flight; pairs of complex values move as one 16-byte access when the
+// layout allows it (VEC).  Entries j >= wv are zero-filled / not stored.
+template <typename T, int ROWS, int W, int NT, bool VEC>
+__device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, long long stride,
+                                            int wv)
+{
+    if constexpr (VEC) {
+        constexpr int W2 = W / 2;
+        constexpr int N = ROWS * W2;
+        constexpr int PER = (N + NT - 1) / NT;
+        float4 r[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = e / W2, j = (e - row * W2) * 2;
+                r[i] = j < wv ? *reinterpret_cast<const float4*>(base + row * stride + j) : make_float4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = e / W2, j = (e - row * W2) * 2;
+                *reinterpret_cast<float4*>(sl + row * W + j) = r[i];
+            }
+        }
+    } else {
+        constexpr int N = ROWS * W;
+        constexpr int PER = (N + NT - 1) / NT;
+        cx<T> r[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = e / W, j = e - row * W;
+                r[i] = j < wv ? base[row * stride + j] : cx<T>{T(0), T(0)};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) sl[e] = r[i];
+        }
+    }
+}
+
+template <typename T, int ROWS, int W, int NT, bool VEC>
+__device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>* __restrict__ base, long long stride,
+                                            int wv)
+{
+    if constexpr (VEC) {
+        constexpr int W2 = W / 2;
+        constexpr int N = ROWS * W2;
+        constexpr int PER = (N + NT - 1) / NT;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = e / W2, j = (e - row * W2) * 2;
+                if (j < wv) *reinterpret_cast<float4*>(base + row * stride + j) = *reinterpret_cast<const float4*>(sl + row * W + j);
+            }
+        }
+    } else {
+        constexpr int N = ROWS * W;
+        constexpr int PER = (N + NT - 1) / NT;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = e / W, j = e - row * W;
+                if (j < wv) base[row * stride + j] = sl[e];
+            }
+        }
+    }
+}
+
+// contiguous spectrum chunk [yl][kz][c] <-> LDS sl[kz*W + yl*C + c]; pairs
+// (c, c+1) move as one 16-byte access for even C in FP32
+template <typename T, int TY, int NZH, int C, int NT>
+__device__ __forceinline__ void zchunk_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ src)
+{
+    constexpr int W = TY * C;
+    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
+    constexpr int G = VEC ? 2 : 1;
+    constexpr int N = TY * NZH * C / G;
+    constexpr int PER = (N + NT - 1) / NT;
+    using V = typename std::conditional<VEC, float4, cx<T>>::type;
+    V r[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) r[i] = reinterpret_cast<const V*>(src)[e];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) {
+            const int f = e * G;
+            const int yl = f / (NZH * C);
+            const int rr = f - yl * (NZH * C);
+            const int kz = rr / C, c = rr - kz * C;
+            *reinterpret_cast<V*>(sl + kz * W + yl * C + c) = r[i];
+        }
+    }
+}
+
+template <typename T, int TY, int NZH, int C, int NT>
+__device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T>* __restrict__ dst)
+{
+    constexpr int W = TY * C;
+    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
+    constexpr int G = VEC ? 2 : 1;
+    constexpr int N = TY * NZH * C / G;
+    constexpr int PER = (N + NT - 1) / NT;
+    using V = typename std::conditional<VEC, float4, cx<T>>::type;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) {
+            const int f = e * G;
+            const int yl = f / (NZH * C);
+            const int rr = f - yl * (NZH * C);
+            const int kz = rr / C, c = rr - kz * C;
+            reinterpret_cast<V*>(dst)[e] = *reinterpret_cast<const V*>(sl + kz * W + yl * C + c);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ z pass
+// Block: solve s, plane x, lines y0..y0+TY-1.  LDS: (LZ+1)*W complex lines
+// (W = TY*C, line l = yl*C + c) + the Nz twiddle table.
+template <typename T, typename PH, int MODE, int LZ, int TY>
+__global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
+{
+    constexpr int C = PH::C;
+    constexpr int W = TY * C;
+    constexpr int NZ = 2 * LZ;
+    constexpr int NZH = LZ + 1;
+    constexpr int NT = ZNT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NT / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = sl + NZH * W;
+
+    const int per = g.Nx * (g.Ny / TY);
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = b / per;
+    const int t = b - s * per;
+    const int x = t / (g.Ny / TY);
+    const int y0 = (t - x * (g.Ny / TY)) * TY;
+    const Ctl& ctl = p.ctl[s];
+    if (ctl.done) return;
+    if (MODE == MODE_CHK_CUR && !ctl.need_cur) return;
+    if (MODE == MODE_CHK_PREV && !ctl.need_prev) return;
+    const int it = ctl.it;
+    const int tid = threadIdx.x;
+
+    const long long vox0 = ((long long)x * g.Ny + y0) * NZ;
+    const long long spec0 = ((long long)x * g.Ny + y0) * NZH * C;
+    const int rv = p.rve_of[s];
+    const uint8_t* ph = p.ph + (long long)rv * p.sPh + vox0;
+    const T* coef = p.coef + (long long)rv * p.sCoef;
+    const T* Uin;
+    T* Uout = nullptr;
+    if (MODE == MODE_FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it & 1, s);
+    else if (MODE == MODE_IUF) { Uin = u_buf(p, it & 1, s); Uout = u_buf(p, (it + 1) & 1, s); }
+    else Uin = u_buf(p, (it + 1) & 1, s);
+    Uin += vox0 * C;
+    if (Uout) Uout += vox0 * C;
+
+    for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
+
+    if (MODE == MODE_IUF) {
+        const cx<T>* src = p.S + (long long)s * p.sS + spec0;
+        zchunk_to_lds<T, TY, NZH, C, NT>(sl, src);
+        __syncthreads();
+        // c2r pre-processing; kz=0 and kz=Nz/2 keep their real parts only
+        constexpr int half = LZ / 2 + 1;
+        for (int e = tid; e < W * half; e += NT) {
+            const int l = e % W, k = e / W;
+            if (k == 0) {
+                const T x0 = sl[l].re, xl = sl[LZ * W + l].re;
+                sl[l] = cx<T>{x0 + xl, x0 - xl};
+            } else {
+                const int kk = LZ - k;
+                const cx<T> Xk = sl[k * W + l], Xc = sl[kk * W + l];
+                cx<T> w = tw[k];
+                w.im = -w.im;
+                sl[k * W + l] = (Xk + cconj(Xc)) + rot90<1>(cmul(w, Xk - cconj(Xc)));
+                if (kk != k) {
+                    cx<T> w2 = tw[kk];
+                    w2.im = -w2.im;
+                    sl[kk * W + l] = (Xc + cconj(Xk)) + rot90<1>(cmul(w2, Xc - cconj(Xk)));
+                }
+            }
+        }
+        __syncthreads();
+        fft<T, LZ, W, NT, 1, 2>(sl, tw);
+    } else {
+        __syncthreads();
+    }
+
+    // per voxel pair (2m, 2m+1) of line yl
+    double acc = 0.0;
+    for (int q = tid; q < TY * LZ; q += NT) {
+        const int yl = q / LZ, m = q - yl * LZ;
+        const int v0 = yl * NZ + 2 * m;
+        T u[2 * C], o0[C], o1[C];
+        vload<T, 2 * C>(Uin + (long long)v0 * C, u);
+        const uint16_t pp = *reinterpret_cast<const uint16_t*>(ph + v0);
+        const T* k0 = coef + (int)(pp & 0xff) * NCOEF;
+        const T* k1 = coef + (int)(pp >> 8) * NCOEF;
+        cx<T>* slot = sl + m * W + yl * C;
+        if (MODE == MODE_IUF) {
+            T d0[C], d1[C], a0[C], a1[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const cx<T> uc = slot[c];
+                d0[c] = uc.re - u[c];
+                d1[c] = uc.im - u[C + c];
+            }
+            PH::accel(k0, d0, a0);
+            PH::accel(k1, d1, a1);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                u[c] += a0[c];
+                u[C + c] += a1[c];
+            }
+            vstore<T, 2 * C>(Uout + (long long)v0 * C, u);
+        }
+        if (MODE == MODE_FWD || MODE == MODE_IUF) {
+            PH::dual(k0, u, o0);
+            PH::dual(k1, u + C, o1);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc += (double)o0[c] + (double)o1[c];
+            PH::polar(k0, u, o0);
+            PH::polar(k1, u + C, o1);
+        } else if (g.strain_base) {
+            PH::dual(k0, u, o0);
+            PH::dual(k1, u + C, o1);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) { o0[c] = u[c]; o1[c] = u[C + c]; }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
+    }
+    if (MODE == MODE_FWD || MODE == MODE_IUF) {
+        const double bs = block_sum(acc, red);
+        if (tid == 0) p.psum[(long long)s * g.nbz + t] = bs;
+    }
+    __syncthreads();
+
+    fft<T, LZ, W, NT, -1, 2>(sl, tw);
+    {
+        constexpr int half = LZ / 2 + 1;
+        for (int e = tid; e < W * half; e += NT) {
+            const int l = e % W, k = e / W;
+            if (k == 0) {
+                const cx<T> z0 = sl[l];
+                sl[l] = cx<T>{z0.re + z0.im, T(0)};
+                sl[LZ * W + l] = cx<T>{z0.re - z0.im, T(0)};
+            } else {
+                const int kk = LZ - k;
+                const cx<T> zk = sl[k * W + l], zc = cconj(sl[kk * W + l]);
+                const cx<T> E = cscale(zk + zc, T(0.5));
+                const cx<T> O = cscale(rot90<-1>(zk - zc), T(0.5));
+                sl[k * W + l] = E + cmul(tw[k], O);
+                if (kk != k) sl[kk * W + l] = cconj(E) + cmul(tw[kk], cconj(O));
+            }
+        }
+    }
+    __syncthreads();
+    {
+        cx<T>* dst = ((MODE == MODE_FWD || MODE == MODE_IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
+        lds_to_zchunk<T, TY, NZH, C, NT>(sl, dst);
+    }
+}
+
+// --------------------------------------------------------- y / x passes
+// KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK>
+__global__ void __launch_bounds__(XYNT) kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+{
+    constexpr int C = PH::C;
+    constexpr int W = TK * C;
+    constexpr int NT = XYNT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NT / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = sl + L * W;
+
+    const int nkt = (g.Nzh + TK - 1) / TK;
+    const int lines_out = KIND == 0 ? g.Nx : g.Ny; // outer index of the tile
+    const int per = lines_out * nkt;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = b / per;
+    const int t = b - s * per;
+    const int o = t / nkt;
+    const int kz0 = (t - o * nkt) * TK;
+    const Ctl& ctl = p.ctl[s];
+    if (ctl.done) return;
+    if (WHICH == BUF_CHK_CUR && !ctl.need_cur) return;
+    if (WHICH == BUF_CHK_PREV && !ctl.need_prev) return;
+    const int nk = min(TK, g.Nzh - kz0);
+    const int wv = nk * C;
+    const int tid = threadIdx.x;
+
+    cx<T>* base;
+    long long estride;
+    const cx<T>* twg;
+    if (KIND == 0) {
+        base = (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + ((long long)o * g.Ny * g.Nzh + kz0) * C;
+        estride = (long long)g.Nzh * C;
+        twg = p.twy;
+    } else {
+        base = (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + ((long long)o * g.Nzh + kz0) * C;
+        estride = (long long)g.Ny * g.Nzh * C;
+        twg = p.twx;
+    }
+    for (int i = tid; i < L; i += NT) tw[i] = twg[i];
+    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
+    rows_to_lds<T, L, W, NT, VEC>(sl, base, estride, wv);
+    __syncthreads();
+    fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1>(sl, tw);
+
+    if (KIND == 1) {
+        const T* prm = prm_all + 2 * p.rve_of[s];
+        const int y = o;
+        for (int f = tid; f < L * nk; f += NT) {
+            const int kx = f / nk, kzl = f - kx * nk;
+            const int kz = kz0 + kzl;
+            cx<T>* v = sl + kx * W + kzl * C;
+            if (kx == 0 && y == 0 && kz == 0) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
+            } else {
+                green_apply<T, PH>(g, prm, kx, y, kz, v);
+            }
+        }
+        __syncthreads();
+        fft<T, L, W, NT, 1, 1>(sl, tw);
+    }
+    if (KIND == 2) {
+        const int y = o;
+        double acc = 0.0;
+        for (int f = tid; f < L * nk; f += NT) {
+            const int kx = f / nk, kzl = f - kx * nk;
+            const int kz = kz0 + kzl;
+            const cx<T>* v = sl + kx * W + kzl * C;
+            double a2[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) a2[c] = (double)v[c].re * (double)v[c].re + (double)v[c].im * (double)v[c].im;
+            if constexpr (C == 6) {
+                if (kz >= 1) {
+                    const double z0 = Elastic::fmap(kz, g.Nz), z1 = Elastic::fmap(y, g.Ny), z2 = Elastic::fmap(kx, g.Nx);
+                    const double q = a2[0] * z0 * z0 + a2[1] * z1 * z1 + a2[2] * z2 * z2 + a2[3] * ((z1 + z1) + (z2 + z2)) +
+                                     a2[4] * ((z0 + z0) + (z2 + z2)) + a2[5] * ((z0 + z0) + (z1 + z1));
+                    acc += sqrt(fabs(q));
+                }
+            } else {
+                const double z0 = fmap_t(kz, g.Nz), z1 = fmap_t(y, g.Ny), z2 = fmap_t(kx, g.Nx);
+                acc += sqrt(a2[0] * z0 * z0 + a2[1] * z1 * z1 + a2[2] * z2 * z2);
+            }
+            if (kx == 0 && y == 0 && kz == 0) {
+                double dd;
+                if constexpr (C == 6) dd = sqrt(a2[0] + a2[1] + a2[2] + 2 * a2[3] + 2 * a2[4] + 2 * a2[5]);
+                else dd = sqrt(a2[0] + a2[1] + a2[2]);
+                p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + g.nbx] = dd;
+            }
+        }
+        const double bs = block_sum(acc, red);
+        if (tid == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + t] = bs;
+        return;
+    }
+    lds_to_rows<T, L, W, NT, VEC>(sl, base, estride, wv);
+}
+
+} // namespace fast
+} // namespace m3d

@@ -70,6 +70,8 @@ struct Geo {
     int nbz, nbx;       // blocks per solve of the z pass and the x pass
     double invN;
     int strain_base;    // elastic: 1 strain base, 0 stress base; thermal: 1 gradient base
+    int fz, fy, fx;     // host only: compile-time specialised kernel used for this axis
+    int gen;            // host only: generic kernels need the odd radices
 };
 
 template <typename T>
@@ -661,6 +663,7 @@ __device__ __forceinline__ double ordered_sum(const double* a, int n, double* re
 }
 
 // after the forward-only z pass: sum of the dual field at the start iterate
+template <int C>
 __global__ void __launch_bounds__(FTHREADS) k_fin0(Geo g, Ctl* ctl, const double* psum)
 {
     __shared__ double red[FTHREADS / 64];
@@ -680,6 +683,7 @@ __global__ void __launch_bounds__(FTHREADS) k_fin0(Geo g, Ctl* ctl, const double
 
 // after the update pass: itNo += 1, diff_sum_field, decide whether the
 // equilibrium check is needed (FFT_analysis3D.cpp:392-416, stop rule :785-792)
+template <int C>
 __global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double* psum)
 {
     __shared__ double red[FTHREADS / 64];
@@ -701,6 +705,7 @@ __global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double
 }
 
 // after the check FFTs: eqVar, diff_avr_eqVar and the stop decision
+template <int C>
 __global__ void __launch_bounds__(FTHREADS) k_finB(Geo g, Ctl* ctl, const double* pchk)
 {
     __shared__ double red[FTHREADS / 64];

@@ -143,3 +143,116 @@ def test_thermal_nphase_fp32_polish():
     err = rel(s.KHomo, r["K"])
     print("thermal 3-phase fp32+polish K rel err", err, [i["iterations"] for i in s.info], r["iters"])
     assert err < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# compile-time specialised kernels: fixed iteration counts against the oracle
+def _fixed_iters_elastic(shape, ids, mat, w, k, precision):
+    o = oracle.Elastic(*shape, "e")
+    o.material(mat, ids)
+    o.run_fixed(w, k)
+    eps_o, sig_o = o.fields()
+    sb = o.info()["strain_base"]
+    o.close()
+    s = m.FFT_Analysis3D(*shape, "e", precision=precision)
+    s.material(mat, ids)
+    s.set_option("cap", k - 1)
+    E = np.zeros(6)
+    E[w] = 0.0005 if sb else 1e6
+    s.setLoad(E, E)
+    info = s.runAnalysis(-1.0, -1.0)
+    assert info["iterations"] == k and info["status"] == 2
+    eps, sig = s.fields()
+    return field_rel(eps, eps_o), field_rel(sig, sig_o)
+
+
+@pytest.mark.parametrize("shape,w,k", [((64, 64, 64), 0, 6), ((96, 96, 96), 3, 4), ((128, 128, 128), 5, 3),
+                                       ((192, 192, 192), 1, 2), ((256, 256, 256), 2, 2), ((32, 64, 128), 4, 5),
+                                       ((48, 48, 48), 0, 6)])
+def test_fast_kernels_fp64_fixed_iterations(shape, w, k):
+    n = max(shape)
+    ids = gen.fibres(n, 4, 0.3, seed=2, length=min(64, n // 2))[: shape[0], : shape[1], : shape[2]].copy()
+    e_eps, e_sig = _fixed_iters_elastic(shape, ids, gen.MAT_C2, w, k, "fp64")
+    assert e_eps < 1e-9 and e_sig < 1e-9, (e_eps, e_sig)
+
+
+@pytest.mark.parametrize("shape,w,k", [((64, 64, 64), 0, 6), ((128, 128, 128), 5, 3), ((256, 256, 256), 2, 2)])
+def test_fast_kernels_fp32_fixed_iterations(shape, w, k):
+    """FP32 fields/FFTs (no polish: the cap ends the run in the FP32 phase):
+    bounded by FP32 round-off of the transforms."""
+    n = max(shape)
+    ids = gen.fibres(n, 4, 0.3, seed=2, length=min(64, n // 2))
+    e_eps, e_sig = _fixed_iters_elastic(shape, ids, gen.MAT_C2, w, k, "fp32_polish")
+    assert e_eps < 2e-5 and e_sig < 2e-5, (e_eps, e_sig)
+
+
+@pytest.mark.parametrize("n,precision", [(64, "fp64"), (128, "fp64"), (128, "fp32_polish")])
+def test_fast_kernels_thermal_fixed_iterations(n, precision):
+    ids = gen.multi_spheres(n, [(2, 0.15, 3, 6), (0, 0.10, 3, 6)], seed=5)
+    k3 = np.array([0.01, 1.0, 1000.0])
+    k = 3
+    o = oracle.Thermal(n, n, n)
+    o.material_n(k3, ids)
+    o.set_cap(k - 1)
+    o.set_load(np.array([0.0, 1.0, 0.0]))
+    o.run(-1.0, -1.0)
+    g_o, q_o = o.fields()
+    o.close()
+    s = m.FFT_analysis3d_thermal(n, n, n, precision=precision)
+    s.material_n(k3, ids)
+    s.set_option("cap", k - 1)
+    s.setLoad([0.0, 1.0, 0.0])
+    info = s.runAnalysis(-1.0, -1.0)
+    assert info["iterations"] == k
+    g, q = s.fields()
+    tol = 1e-9 if precision == "fp64" else 2e-5
+    assert field_rel(g, g_o) < tol and field_rel(q, q_o) < tol
+
+
+# ---------------------------------------------------------------------------
+# size-independent properties at full size: exact laminate solutions
+@pytest.mark.parametrize("n,precision", [(512, "fp32_polish"), (256, "fp64")])
+def test_thermal_laminate_exact_full_size(n, precision):
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[: n // 4] = 0  # layers normal to x (tensor axis 2)
+    km, ki = 1.0, 20.0
+    s = m.FFT_analysis3d_thermal(n, n, n, precision=precision)
+    s.material(km, ki, ids)
+    s.homogenization(1e-7, 1e-7)
+    print("laminate", n, precision, [(i["iterations"], i["status"]) for i in s.info], s.KHomo.diagonal())
+    assert all(i["status"] == 1 for i in s.info)
+    f = 0.25
+    par, ser = f * ki + (1 - f) * km, 1 / (f / ki + (1 - f) / km)
+    K = s.KHomo
+    assert abs(K[0, 0] - par) < 1e-6 * par and abs(K[1, 1] - par) < 1e-6 * par
+    assert abs(K[2, 2] - ser) < 1e-6 * ser
+
+
+def test_elastic_laminate_exact_256_fp32_polish():
+    from test_oracle import laminate_exact
+
+    n = 256
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[:, :, : 3 * n // 8] = 0  # layers normal to z (tensor axis 0)
+    mat = gen.MAT_C2
+    s = m.FFT_Analysis3D(n, n, n, "e", precision="fp32_polish")
+    s.material(mat, ids)
+    # the stop rule bounds the change per iteration, not the distance to the
+    # fixed point: a tight tolerance makes the converged iterate exact to 1e-6
+    s.homogenization(1e-10, 1e-10)
+    assert all(i["status"] == 1 for i in s.info)
+    print("elastic laminate 256", [(i["iterations"], i["status"]) for i in s.info])
+    f0 = 3 / 8
+    sb = s.strainBase
+    ep = np.zeros((6, 6)); sg = np.zeros((6, 6))
+    for w in range(6):
+        if sb:
+            E = np.eye(6)[w] * 0.0005
+        else:
+            Cl = np.stack([laminate_exact(mat[:, :2], [f0, 1 - f0], 0, b, sb)[1] for b in np.eye(6)], 1)
+            E = np.linalg.solve(Cl, np.eye(6)[w] * 1e6)
+        e, sgm = laminate_exact(mat[:, :2], [f0, 1 - f0], 0, E, sb)
+        ep[:, w] = e
+        sg[w, :] = sgm
+    Cref = np.linalg.inv(ep) @ sg if sb else np.linalg.inv(np.linalg.inv(sg) @ ep)
+    assert rel(s.CHomo, Cref) < 1e-6
