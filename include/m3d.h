@@ -100,8 +100,10 @@ int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0,
 
 /* Tuning knobs (no reference counterpart): key "polish_factor" (the FP32
  * phase stops at max(tol*factor, fp32_floor), default 10), "fp32_floor"
- * (default 1e-6), "cap" (iteration cap, default 2000 elastic / 500 thermal),
- * "chunk" (iterations per graph replay, 0 = automatic). */
+ * (default 1e-6), "rollback" (FP32 iterations redone in FP64 when the switch
+ * iterate already satisfies the stop rule within 2 x tol, default 5), "cap"
+ * (iteration cap, default 2000 elastic / 500 thermal), "chunk" (iterations
+ * per graph replay, 0 = automatic). */
 int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 
 /* Kernel timing for measurement tools (no reference counterpart).  Times

@@ -1,0 +1,185 @@
+// m3d_compat.hpp -- header-only C++ wrapper that re-exposes the reference's
+// solver classes on top of the C-ABI in m3d.h.
+//
+//   FFT_Analysis3D          -- include/FFT_analysis3D.hpp:36-215
+//   FFT_analysis3d_thermal  -- include/FFT_thermal3D.hpp:37-154
+//
+// Method names, argument order and meaning are the reference's.  The matrix
+// arguments are templates: anything with data()/rows()/cols() in the
+// reference's column-major order works, Eigen's MatrixDdd / MatrixIdd
+// included (no Eigen dependency here).  Public result members are plain
+// std::vector<double> in column-major order (CHomo, SHomo, epHomo, sigHomo:
+// 6x6; KHomo, RHomo, tempGradHomo, heatFluxHomo: 3x3; epsilon/sigma: 6 x N).
+// Failures throw m3d::Error (the reference exits or terminates instead,
+// e.g. src/3d/FFT_analysis3D.cpp:672-673).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "m3d.h"
+
+namespace m3d {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+class FFT_Analysis3D {
+  public:
+    // FFT_Analysis3D(int Nx1, int Ny1, int Nz1, string mater_behav), FFT_analysis3D.hpp:196
+    FFT_Analysis3D(int Nx1, int Ny1, int Nz1, const std::string& mater_behav, int precision = M3D_FP32_POLISH)
+        : Nx(Nx1), Ny(Ny1), Nz(Nz1), xyz(Nx1 * Ny1 * Nz1)
+    {
+        int rc = m3d_elastic_create(Nx1, Ny1, Nz1, mater_behav.c_str(), precision, &h_);
+        if (rc != M3D_OK) throw Error(rc, "m3d_elastic_create failed");
+    }
+    ~FFT_Analysis3D() { m3d_elastic_destroy(h_); }
+    FFT_Analysis3D(const FFT_Analysis3D&) = delete;
+    FFT_Analysis3D& operator=(const FFT_Analysis3D&) = delete;
+
+    // void material(MatrixDdd& mat, MatrixIdd& RVE1, int noPhases), FFT_analysis3D.hpp:200
+    template <class Mat, class Ids>
+    void material(const Mat& mat, const Ids& RVE1, int noPhases)
+    {
+        std::vector<int32_t> ids(RVE1.data(), RVE1.data() + xyz);
+        // mat is noPhases x 7 column-major; copy the first noPhases rows
+        std::vector<double> m((size_t)noPhases * 7);
+        for (int j = 0; j < 7; ++j)
+            for (int i = 0; i < noPhases; ++i) m[i + (size_t)j * noPhases] = mat.data()[i + (size_t)j * mat.rows()];
+        chk(m3d_elastic_material(h_, m.data(), ids.data(), noPhases));
+        vol_fractions.assign(noPhases, 0.0);
+        int sb = 1;
+        chk(m3d_elastic_info(h_, vol_fractions.data(), &sb, &la0, &mu0));
+        strainBase = sb != 0;
+    }
+
+    // void setLoad(MatrixDdd& E1, MatrixDdd& X0), FFT_analysis3D.hpp:202
+    template <class V>
+    void setLoad(const V& E1, const V& X01)
+    {
+        for (int i = 0; i < 6; ++i) { E[i] = E1.data()[i]; X0[i] = X01.data()[i]; }
+        chk(m3d_elastic_set_load(h_, E, X0));
+    }
+
+    // void runAnalysis(double tol_field, double tol_eq, int nInc1), FFT_analysis3D.hpp:206
+    void runAnalysis(double tol_field, double tol_eq, int nInc1)
+    {
+        chk(m3d_elastic_run(h_, tol_field, tol_eq, nInc1, &info[0]));
+        fetch_fields(0);
+    }
+
+    // double homogenization(double, double, string), FFT_analysis3D.hpp:209
+    double homogenization(double tol_field, double tol_eq, const std::string& /*outputName*/ = "")
+    {
+        CHomo.resize(36); SHomo.resize(36); epHomo.resize(36); sigHomo.resize(36);
+        double ret = 0;
+        chk(m3d_elastic_homogenization(h_, tol_field, tol_eq, CHomo.data(), SHomo.data(), epHomo.data(),
+                                       sigHomo.data(), &ret, info));
+        E11 = 1. / SHomo[0]; E22 = 1. / SHomo[7]; E33 = 1. / SHomo[14];
+        fetch_fields(5); // the reference keeps the fields of the last load case
+        return ret;
+    }
+
+    void set_option(const char* key, double v) { chk(m3d_elastic_set_option(h_, key, v)); }
+
+    // public members of the reference class
+    int Nx, Ny, Nz, xyz;
+    double E[6] = {0, 0, 0, 0, 0, 0}, X0[6] = {0, 0, 0, 0, 0, 0};
+    double la0 = 0, mu0 = 0, E11 = 0, E22 = 0, E33 = 0;
+    bool strainBase = true;
+    std::vector<double> vol_fractions, CHomo, SHomo, epHomo, sigHomo;
+    std::vector<double> epsilon, sigma; // 6 x N column-major
+    m3d_run_info info[6] = {};
+
+  private:
+    void chk(int rc)
+    {
+        if (rc != M3D_OK) throw Error(rc, m3d_elastic_last_error(h_));
+    }
+    void fetch_fields(int lc)
+    {
+        epsilon.resize((size_t)6 * xyz);
+        sigma.resize((size_t)6 * xyz);
+        chk(m3d_elastic_get_fields(h_, lc, epsilon.data(), sigma.data()));
+    }
+    m3d_elastic* h_ = nullptr;
+};
+
+class FFT_analysis3d_thermal {
+  public:
+    // FFT_analysis3d_thermal(int Nx1, int Ny1, int Nz1), FFT_thermal3D.hpp:133
+    FFT_analysis3d_thermal(int Nx1, int Ny1, int Nz1, int precision = M3D_FP32_POLISH)
+        : Nx(Nx1), Ny(Ny1), Nz(Nz1), xyz(Nx1 * Ny1 * Nz1)
+    {
+        int rc = m3d_thermal_create(Nx1, Ny1, Nz1, precision, &h_);
+        if (rc != M3D_OK) throw Error(rc, "m3d_thermal_create failed");
+    }
+    ~FFT_analysis3d_thermal() { m3d_thermal_destroy(h_); }
+    FFT_analysis3d_thermal(const FFT_analysis3d_thermal&) = delete;
+    FFT_analysis3d_thermal& operator=(const FFT_analysis3d_thermal&) = delete;
+
+    // void material(double km, double ki, MatrixIdd& RVE1), FFT_thermal3D.hpp:138
+    template <class Ids>
+    void material(double km, double ki, const Ids& RVE1)
+    {
+        std::vector<int32_t> ids(RVE1.data(), RVE1.data() + xyz);
+        chk(m3d_thermal_material(h_, km, ki, ids.data()));
+        if (have_load_) chk(m3d_thermal_set_load(h_, E));
+    }
+
+    // void setLoad(MatrixDdd& E1), FFT_thermal3D.hpp:140.  The reference example
+    // calls it before material() (example/3d_thermal/main.cpp:94-95); the load
+    // is kept and re-applied.
+    template <class V>
+    void setLoad(const V& E1)
+    {
+        for (int i = 0; i < 3; ++i) E[i] = E1.data()[i];
+        have_load_ = true;
+        chk(m3d_thermal_set_load(h_, E));
+    }
+
+    // void runAnalysis(double tol_field, double tol_eq), FFT_thermal3D.hpp:145
+    void runAnalysis(double tol_field, double tol_eq)
+    {
+        chk(m3d_thermal_run(h_, tol_field, tol_eq, &info[0]));
+        fetch_fields(0);
+    }
+
+    // void homogenization(double, double, string), FFT_thermal3D.hpp:148
+    void homogenization(double tol_field, double tol_eq, const std::string& /*filename*/ = "")
+    {
+        KHomo.resize(9); RHomo.resize(9); tempGradHomo.resize(9); heatFluxHomo.resize(9);
+        chk(m3d_thermal_homogenization(h_, tol_field, tol_eq, KHomo.data(), RHomo.data(), tempGradHomo.data(),
+                                       heatFluxHomo.data(), info));
+        K11 = KHomo[0]; K22 = KHomo[4]; K33 = KHomo[8];
+        fetch_fields(2);
+    }
+
+    void set_option(const char* key, double v) { chk(m3d_thermal_set_option(h_, key, v)); }
+
+    int Nx, Ny, Nz, xyz;
+    double E[3] = {0, 0, 0};
+    double K11 = 0, K22 = 0, K33 = 0;
+    std::vector<double> KHomo, RHomo, tempGradHomo, heatFluxHomo;
+    std::vector<double> tempGradient, heatFlux; // 3 x N column-major
+    m3d_run_info info[3] = {};
+
+  private:
+    void chk(int rc)
+    {
+        if (rc != M3D_OK) throw Error(rc, m3d_thermal_last_error(h_));
+    }
+    void fetch_fields(int lc)
+    {
+        tempGradient.resize((size_t)3 * xyz);
+        heatFlux.resize((size_t)3 * xyz);
+        chk(m3d_thermal_get_fields(h_, lc, tempGradient.data(), heatFlux.data()));
+    }
+    m3d_thermal* h_ = nullptr;
+    bool have_load_ = false;
+};
+
+} // namespace m3d

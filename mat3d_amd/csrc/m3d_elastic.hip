@@ -132,6 +132,7 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
         std::string k = key ? key : "";
         if (k == "polish_factor" && value >= 1.0) h->eng->polish_factor = value;
         else if (k == "fp32_floor" && value >= 0.0) h->eng->fp32_floor = value;
+        else if (k == "rollback" && value >= 0 && value <= 64) h->eng->rollback = (int)value;
         else if (k == "cap" && value >= 1) h->eng->cap = (int)value;
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};

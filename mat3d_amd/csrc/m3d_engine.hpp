@@ -229,8 +229,9 @@ struct Engine {
     int cap_solves = 0;
     long long N = 0, H = 0;
     // device buffers
-    float* U32[2] = {nullptr, nullptr};
-    double* U64[2] = {nullptr, nullptr};
+    float* U32 = nullptr;  // ring of rollback+1 FP32 iterates x solves
+    double* U64 = nullptr; // two FP64 iterates x solves
+    int ring32 = 0;        // ring size the FP32 buffer was allocated with
     unsigned char* Sbuf = nullptr;
     unsigned char* Kbuf = nullptr;
     double* psum = nullptr;
@@ -253,6 +254,8 @@ struct Engine {
     std::vector<int> final_prec; // per solve: 0 fp32, 1 fp64
     std::vector<m3d_run_info> info;
     double polish_factor = 10.0;
+    int rollback = 5;
+    double rollback_margin = 2.0;
     double fp32_floor = 1e-6;
     int cap = 2000;
     int chunk = 0;
@@ -282,12 +285,10 @@ struct Engine {
 
     void free_solves()
     {
-        for (int b = 0; b < 2; ++b) {
-            if (U32[b]) hipFree(U32[b]);
-            if (U64[b]) hipFree(U64[b]);
-            U32[b] = nullptr;
-            U64[b] = nullptr;
-        }
+        if (U32) hipFree(U32);
+        if (U64) hipFree(U64);
+        U32 = nullptr;
+        U64 = nullptr;
         if (Sbuf) hipFree(Sbuf);
         if (Kbuf) hipFree(Kbuf);
         if (psum) hipFree(psum);
@@ -332,18 +333,19 @@ struct Engine {
         }
     }
 
+    int ring_needed() const { return std::max(2, rollback + 1); }
+
     void ensure_solves(int n)
     {
-        if (n <= cap_solves) return;
+        if (n <= cap_solves && (precision != M3D_FP32_POLISH || ring32 == ring_needed())) return;
         free_solves();
         const size_t nc = (size_t)N * C;
         const size_t hc = (size_t)H * C;
         if (precision == M3D_FP32_POLISH) {
-            alloc(&U32[0], sizeof(float) * nc * n);
-            alloc(&U32[1], sizeof(float) * nc * n);
+            ring32 = ring_needed();
+            alloc(&U32, sizeof(float) * nc * n * ring32);
         }
-        alloc(&U64[0], sizeof(double) * nc * n);
-        alloc(&U64[1], sizeof(double) * nc * n);
+        alloc(&U64, sizeof(double) * nc * n * 2);
         alloc(&Sbuf, sizeof(cx<double>) * hc * n);
         alloc(&Kbuf, sizeof(cx<double>) * hc * n);
         const int nbz = std::max(g32.nbz, g64.nbz), nbx = std::max(g32.nbx, g64.nbx);
@@ -364,21 +366,22 @@ struct Engine {
         std::memset(&p, 0, sizeof p);
         const size_t nc = (size_t)N * C;
         if constexpr (sizeof(T) == 4) {
-            p.U0 = U32[0];
-            p.U1 = U32[1];
+            p.U = U32;
+            p.ring = ring32;
             p.coef = coef32;
             p.twz = tw32[0];
             p.twy = tw32[1];
             p.twx = tw32[2];
         } else {
-            p.U0 = U64[0];
-            p.U1 = U64[1];
+            p.U = U64;
+            p.ring = 2;
             p.coef = coef64;
             p.twz = tw64[0];
             p.twy = tw64[1];
             p.twx = tw64[2];
         }
         p.sU = (long long)nc;
+        p.sB = (long long)nc * cap_solves;
         p.S = reinterpret_cast<cx<T>*>(Sbuf);
         p.K = reinterpret_cast<cx<T>*>(Kbuf);
         p.sS = (long long)H * C * (sizeof(cx<double>) / sizeof(cx<T>));
@@ -604,25 +607,36 @@ struct Engine {
             Ptr<float> pf = ptrs<float>(gf);
             hipLaunchKernelGGL((k_fill_load<float, C>), dim3(1024), dim3(256), 0, st, gf, pf, 0);
             run_phase<float>(gf, hctl);
-            std::vector<int> it32(nsolve);
+            // roll back `rollback` iterations and redo them in FP64 under the
+            // exact rule: the FP32 rule fires at or before the reference's stop
+            // iteration, so every candidate stop iterate is an FP64 iterate
+            std::vector<int> it32(nsolve), jr(nsolve, 0);
             for (int s = 0; s < nsolve; ++s) {
                 it32[s] = hctl[s].it;
                 info[s].iters_fp32 = hctl[s].it;
                 if (hctl[s].done == 1) {
+                    // borderline switch iterate: both criteria already within
+                    // rollback_margin x tol in FP32 -> the reference may stop
+                    // right here, so the last iterations are redone in FP64
+                    const bool border = hctl[s].dsum < rollback_margin * tol_f && hctl[s].deq < rollback_margin * tol_eq;
+                    jr[s] = border ? std::max(0, hctl[s].it - rollback) : hctl[s].it;
+                    hctl[s].it = jr[s];
                     hctl[s].done = 0;
+                    hctl[s].need_cur = hctl[s].need_prev = 0;
+                    hctl[s].eq_it = -1;
                     hctl[s].tol_f = tol_f;
                     hctl[s].tol_eq = tol_eq;
                 } else {
+                    jr[s] = hctl[s].it;
                     final_prec[s] = 0; // iteration cap reached in FP32: keep that state
                 }
             }
-            hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, U32[0], U32[1], (long long)N * C, U64[0],
-                               U64[1], (long long)N * C, ctl);
             HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, ptrs<float>(gf), ptrs<double>(gd), ctl);
             bool any = false;
             for (int s = 0; s < nsolve; ++s) any = any || (hctl[s].done == 0);
             if (any) run_phase<double>(gd, hctl);
-            for (int s = 0; s < nsolve; ++s) info[s].iters_fp64 = hctl[s].it - it32[s];
+            for (int s = 0; s < nsolve; ++s) info[s].iters_fp64 = hctl[s].it - jr[s];
         } else {
             Ptr<double> pd = ptrs<double>(gd);
             hipLaunchKernelGGL((k_fill_load<double, C>), dim3(1024), dim3(256), 0, st, gd, pd, 0);

@@ -21,6 +21,15 @@
 namespace m3d {
 namespace fast {
 
+// native vector types for register staging (HIP's float4 is a struct and
+// arrays of it are not promoted to registers when copied)
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+// one complex value (8 or 16 bytes) or a pair of FP32 complex values (16 bytes)
+template <typename T, bool PAIR>
+using stage_t = typename std::conditional<PAIR, vf4, typename std::conditional<sizeof(T) == 4, vf2, vd2>::type>::type;
+
 // ---------------------------------------------------------------- planning
 constexpr int n_twos(int L) { return (L % 2 == 0) ? 1 + n_twos(L / 2) : 0; }
 constexpr int odd_part(int L) { return (L % 2 == 0) ? odd_part(L / 2) : L; }
@@ -133,35 +142,45 @@ __device__ __forceinline__ void fft(cx<T>* __restrict__ s, const cx<T>* __restri
     }
 }
 
-// vector load/store of n contiguous values (16-byte chunks when aligned, else 8)
+// vector load/store of n contiguous values into a register array (16-byte
+// chunks when n*sizeof(T) allows it, else 8); components are moved one by one
+// so the register array is never addressed as memory
 template <typename T, int n>
-__device__ __forceinline__ void vload(const T* __restrict__ g, T* r)
+__device__ __forceinline__ void vload(const T* __restrict__ g, T (&r)[n])
 {
-    if constexpr ((n * sizeof(T)) % 16 == 0) {
+    if constexpr (sizeof(T) == 4 && n % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < (int)(n * sizeof(T) / 16); ++i) {
+        for (int i = 0; i < n / 4; ++i) {
             const float4 q = reinterpret_cast<const float4*>(g)[i];
-            reinterpret_cast<float4*>(r)[i] = q;
+            r[4 * i] = q.x; r[4 * i + 1] = q.y; r[4 * i + 2] = q.z; r[4 * i + 3] = q.w;
+        }
+    } else if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int i = 0; i < n / 2; ++i) {
+            const float2 q = reinterpret_cast<const float2*>(g)[i];
+            r[2 * i] = q.x; r[2 * i + 1] = q.y;
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < (int)(n * sizeof(T) / 8); ++i) {
-            const float2 q = reinterpret_cast<const float2*>(g)[i];
-            reinterpret_cast<float2*>(r)[i] = q;
+        for (int i = 0; i < n / 2; ++i) {
+            const double2 q = reinterpret_cast<const double2*>(g)[i];
+            r[2 * i] = q.x; r[2 * i + 1] = q.y;
         }
     }
 }
 template <typename T, int n>
-__device__ __forceinline__ void vstore(T* __restrict__ g, const T* r)
+__device__ __forceinline__ void vstore(T* __restrict__ g, const T (&r)[n])
 {
-    if constexpr ((n * sizeof(T)) % 16 == 0) {
+    if constexpr (sizeof(T) == 4 && n % 4 == 0) {
 #pragma unroll
-        for (int i = 0; i < (int)(n * sizeof(T) / 16); ++i)
-            reinterpret_cast<float4*>(g)[i] = reinterpret_cast<const float4*>(r)[i];
+        for (int i = 0; i < n / 4; ++i)
+            reinterpret_cast<float4*>(g)[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+    } else if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int i = 0; i < n / 2; ++i) reinterpret_cast<float2*>(g)[i] = make_float2(r[2 * i], r[2 * i + 1]);
     } else {
 #pragma unroll
-        for (int i = 0; i < (int)(n * sizeof(T) / 8); ++i)
-            reinterpret_cast<float2*>(g)[i] = reinterpret_cast<const float2*>(r)[i];
+        for (int i = 0; i < n / 2; ++i) reinterpret_cast<double2*>(g)[i] = make_double2(r[2 * i], r[2 * i + 1]);
     }
 }
 
@@ -175,43 +194,26 @@ template <typename T, int ROWS, int W, int NT, bool VEC>
 __device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, long long stride,
                                             int wv)
 {
-    if constexpr (VEC) {
-        constexpr int W2 = W / 2;
-        constexpr int N = ROWS * W2;
-        constexpr int PER = (N + NT - 1) / NT;
-        float4 r[PER];
+    using V = stage_t<T, VEC>;
+    constexpr int G = VEC ? 2 : 1;
+    constexpr int WG = W / G;
+    constexpr int N = ROWS * WG;
+    constexpr int PER = (N + NT - 1) / NT;
+    V r[PER];
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) {
-                const int row = e / W2, j = (e - row * W2) * 2;
-                r[i] = j < wv ? *reinterpret_cast<const float4*>(base + row * stride + j) : make_float4(0, 0, 0, 0);
-            }
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) {
+            const int row = e / WG, j = (e - row * WG) * G;
+            r[i] = j < wv ? *reinterpret_cast<const V*>(base + row * stride + j) : V(0);
         }
+    }
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) {
-                const int row = e / W2, j = (e - row * W2) * 2;
-                *reinterpret_cast<float4*>(sl + row * W + j) = r[i];
-            }
-        }
-    } else {
-        constexpr int N = ROWS * W;
-        constexpr int PER = (N + NT - 1) / NT;
-        cx<T> r[PER];
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) {
-                const int row = e / W, j = e - row * W;
-                r[i] = j < wv ? base[row * stride + j] : cx<T>{T(0), T(0)};
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) sl[e] = r[i];
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) {
+            const int row = e / WG, j = (e - row * WG) * G;
+            *reinterpret_cast<V*>(sl + row * W + j) = r[i];
         }
     }
 }
@@ -220,58 +222,59 @@ template <typename T, int ROWS, int W, int NT, bool VEC>
 __device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>* __restrict__ base, long long stride,
                                             int wv)
 {
-    if constexpr (VEC) {
-        constexpr int W2 = W / 2;
-        constexpr int N = ROWS * W2;
-        constexpr int PER = (N + NT - 1) / NT;
+    using V = stage_t<T, VEC>;
+    constexpr int G = VEC ? 2 : 1;
+    constexpr int WG = W / G;
+    constexpr int N = ROWS * WG;
+    constexpr int PER = (N + NT - 1) / NT;
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) {
-                const int row = e / W2, j = (e - row * W2) * 2;
-                if (j < wv) *reinterpret_cast<float4*>(base + row * stride + j) = *reinterpret_cast<const float4*>(sl + row * W + j);
-            }
-        }
-    } else {
-        constexpr int N = ROWS * W;
-        constexpr int PER = (N + NT - 1) / NT;
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (N % NT == 0 || e < N) {
-                const int row = e / W, j = e - row * W;
-                if (j < wv) base[row * stride + j] = sl[e];
-            }
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (N % NT == 0 || e < N) {
+            const int row = e / WG, j = (e - row * WG) * G;
+            if (j < wv) *reinterpret_cast<V*>(base + row * stride + j) = *reinterpret_cast<const V*>(sl + row * W + j);
         }
     }
 }
 
 // contiguous spectrum chunk [yl][kz][c] <-> LDS sl[kz*W + yl*C + c]; pairs
-// (c, c+1) move as one 16-byte access for even C in FP32
+// (c, c+1) move as one 16-byte access for even C in FP32.  The load is split
+// in an issue step (global -> registers) and a commit step (registers ->
+// LDS) so other loads can be issued in between.
 template <typename T, int TY, int NZH, int C, int NT>
-__device__ __forceinline__ void zchunk_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ src)
+struct ZChunk {
+    static constexpr int W = TY * C;
+    static constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
+    static constexpr int G = VEC ? 2 : 1;
+    static constexpr int N = TY * NZH * C / G;
+    static constexpr int PER = (N + NT - 1) / NT;
+    using V = stage_t<T, VEC>;
+};
+
+template <typename Z, typename V>
+__device__ __forceinline__ void zchunk_issue(V (&r)[Z::PER], const void* __restrict__ src)
 {
-    constexpr int W = TY * C;
-    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
-    constexpr int G = VEC ? 2 : 1;
-    constexpr int N = TY * NZH * C / G;
-    constexpr int PER = (N + NT - 1) / NT;
-    using V = typename std::conditional<VEC, float4, cx<T>>::type;
-    V r[PER];
+    constexpr int NT = ZNT;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
+    for (int i = 0; i < Z::PER; ++i) {
         const int e = threadIdx.x + i * NT;
-        if (N % NT == 0 || e < N) r[i] = reinterpret_cast<const V*>(src)[e];
+        if (Z::N % NT == 0 || e < Z::N) r[i] = reinterpret_cast<const V*>(src)[e];
     }
+}
+
+template <typename Z, int NZH, int C, typename T, typename V>
+__device__ __forceinline__ void zchunk_commit(const V (&r)[Z::PER], cx<T>* __restrict__ sl)
+{
+    constexpr int NT = ZNT;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
+    for (int i = 0; i < Z::PER; ++i) {
         const int e = threadIdx.x + i * NT;
-        if (N % NT == 0 || e < N) {
-            const int f = e * G;
+        if (Z::N % NT == 0 || e < Z::N) {
+            const int f = e * Z::G;
             const int yl = f / (NZH * C);
             const int rr = f - yl * (NZH * C);
             const int kz = rr / C, c = rr - kz * C;
-            *reinterpret_cast<V*>(sl + kz * W + yl * C + c) = r[i];
+            *reinterpret_cast<V*>(sl + kz * Z::W + yl * C + c) = r[i];
         }
     }
 }
@@ -334,17 +337,35 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
     const T* coef = p.coef + (long long)rv * p.sCoef;
     const T* Uin;
     T* Uout = nullptr;
-    if (MODE == MODE_FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it & 1, s);
-    else if (MODE == MODE_IUF) { Uin = u_buf(p, it & 1, s); Uout = u_buf(p, (it + 1) & 1, s); }
-    else Uin = u_buf(p, (it + 1) & 1, s);
+    if (MODE == MODE_FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
+    else if (MODE == MODE_IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
+    else Uin = u_buf(p, it - 1, s);
     Uin += vox0 * C;
     if (Uout) Uout += vox0 * C;
 
+    // issue every global load of the tile first: spectrum chunk, then the
+    // unknowns and phase ids of this thread's voxel pairs
+    constexpr int NPAIR = TY * LZ;
+    constexpr int QI = (NPAIR + NT - 1) / NT;
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
+    using ZC = ZChunk<T, TY, NZH, C, NT>;
+    typename ZC::V chunk[ZC::PER];
+    if (MODE == MODE_IUF) zchunk_issue<ZC>(chunk, p.S + (long long)s * p.sS + spec0);
+    T u[QI][2 * C];
+    uint16_t pp[QI];
+#pragma unroll
+    for (int i = 0; i < QI; ++i) {
+        const int q = tid + i * NT;
+        if (NPAIR % NT == 0 || q < NPAIR) {
+            const int yl = q / LZ, m = q - yl * LZ;
+            const int v0 = yl * NZ + 2 * m;
+            vload<T, 2 * C>(Uin + (long long)v0 * C, u[i]);
+            pp[i] = *reinterpret_cast<const uint16_t*>(ph + v0);
+        }
+    }
 
     if (MODE == MODE_IUF) {
-        const cx<T>* src = p.S + (long long)s * p.sS + spec0;
-        zchunk_to_lds<T, TY, NZH, C, NT>(sl, src);
+        zchunk_commit<ZC, NZH, C>(chunk, sl);
         __syncthreads();
         // c2r pre-processing; kz=0 and kz=Nz/2 keep their real parts only
         constexpr int half = LZ / 2 + 1;
@@ -374,48 +395,55 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
 
     // per voxel pair (2m, 2m+1) of line yl
     double acc = 0.0;
-    for (int q = tid; q < TY * LZ; q += NT) {
-        const int yl = q / LZ, m = q - yl * LZ;
-        const int v0 = yl * NZ + 2 * m;
-        T u[2 * C], o0[C], o1[C];
-        vload<T, 2 * C>(Uin + (long long)v0 * C, u);
-        const uint16_t pp = *reinterpret_cast<const uint16_t*>(ph + v0);
-        const T* k0 = coef + (int)(pp & 0xff) * NCOEF;
-        const T* k1 = coef + (int)(pp >> 8) * NCOEF;
-        cx<T>* slot = sl + m * W + yl * C;
-        if (MODE == MODE_IUF) {
-            T d0[C], d1[C], a0[C], a1[C];
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const cx<T> uc = slot[c];
-                d0[c] = uc.re - u[c];
-                d1[c] = uc.im - u[C + c];
+    for (int i = 0; i < QI; ++i) {
+        const int q = tid + i * NT;
+        if (NPAIR % NT == 0 || q < NPAIR) {
+            const int yl = q / LZ, m = q - yl * LZ;
+            const int v0 = yl * NZ + 2 * m;
+            T u0[C], u1[C], o0[C], o1[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) { u0[c] = u[i][c]; u1[c] = u[i][C + c]; }
+            const T* k0 = coef + (int)(pp[i] & 0xff) * NCOEF;
+            const T* k1 = coef + (int)(pp[i] >> 8) * NCOEF;
+            cx<T>* slot = sl + m * W + yl * C;
+            if (MODE == MODE_IUF) {
+                T d0[C], d1[C], a0[C], a1[C];
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const cx<T> uc = slot[c];
+                    d0[c] = uc.re - u0[c];
+                    d1[c] = uc.im - u1[c];
+                }
+                PH::accel(k0, d0, a0);
+                PH::accel(k1, d1, a1);
+                T un[2 * C];
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    u0[c] += a0[c];
+                    u1[c] += a1[c];
+                    un[c] = u0[c];
+                    un[C + c] = u1[c];
+                }
+                vstore<T, 2 * C>(Uout + (long long)v0 * C, un);
             }
-            PH::accel(k0, d0, a0);
-            PH::accel(k1, d1, a1);
+            if (MODE == MODE_FWD || MODE == MODE_IUF) {
+                PH::dual(k0, u0, o0);
+                PH::dual(k1, u1, o1);
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                u[c] += a0[c];
-                u[C + c] += a1[c];
+                for (int c = 0; c < C; ++c) acc += (double)o0[c] + (double)o1[c];
+                PH::polar(k0, u0, o0);
+                PH::polar(k1, u1, o1);
+            } else if (g.strain_base) {
+                PH::dual(k0, u0, o0);
+                PH::dual(k1, u1, o1);
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; ++c) { o0[c] = u0[c]; o1[c] = u1[c]; }
             }
-            vstore<T, 2 * C>(Uout + (long long)v0 * C, u);
+#pragma unroll
+            for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
         }
-        if (MODE == MODE_FWD || MODE == MODE_IUF) {
-            PH::dual(k0, u, o0);
-            PH::dual(k1, u + C, o1);
-#pragma unroll
-            for (int c = 0; c < C; ++c) acc += (double)o0[c] + (double)o1[c];
-            PH::polar(k0, u, o0);
-            PH::polar(k1, u + C, o1);
-        } else if (g.strain_base) {
-            PH::dual(k0, u, o0);
-            PH::dual(k1, u + C, o1);
-        } else {
-#pragma unroll
-            for (int c = 0; c < C; ++c) { o0[c] = u[c]; o1[c] = u[C + c]; }
-        }
-#pragma unroll
-        for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
     }
     if (MODE == MODE_FWD || MODE == MODE_IUF) {
         const double bs = block_sum(acc, red);

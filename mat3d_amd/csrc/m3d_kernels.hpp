@@ -76,9 +76,10 @@ struct Geo {
 
 template <typename T>
 struct Ptr {
-    T* U0;
-    T* U1;
+    T* U;                   // ring of `ring` iterate buffers, each holding every solve
     long long sU;           // per-solve stride of U (reals)
+    long long sB;           // stride between ring buffers (reals)
+    int ring;               // iterate j lives in buffer j mod ring
     cx<T>* S;               // main spectrum
     cx<T>* K;               // check spectrum
     long long sS;           // per-solve stride of S and K (complex)
@@ -239,9 +240,11 @@ __device__ __forceinline__ double block_sum(double v, double* red)
 }
 
 template <typename T>
-__device__ __forceinline__ T* u_buf(const Ptr<T>& p, int b, int s)
+__device__ __forceinline__ T* u_buf(const Ptr<T>& p, int it, int s)
 {
-    return (b ? p.U1 : p.U0) + (long long)s * p.sU;
+    int b = it % p.ring;
+    if (b < 0) b += p.ring;
+    return p.U + (long long)b * p.sB + (long long)s * p.sU;
 }
 
 // --------------------------------------------------------------- z pass
@@ -277,10 +280,10 @@ __global__ void __launch_bounds__(ZTHREADS) k_zpass(Geo g, Ptr<T> p)
 
     const T* Uin;
     T* Uout = nullptr;
-    if (MODE == MODE_FWD) Uin = u_buf(p, it & 1, s);
-    else if (MODE == MODE_IUF) { Uin = u_buf(p, it & 1, s); Uout = u_buf(p, (it + 1) & 1, s); }
-    else if (MODE == MODE_CHK_CUR) Uin = u_buf(p, it & 1, s);
-    else Uin = u_buf(p, (it + 1) & 1, s);
+    if (MODE == MODE_FWD) Uin = u_buf(p, it, s);
+    else if (MODE == MODE_IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
+    else if (MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
+    else Uin = u_buf(p, it - 1, s);
     Uin += vox0 * C;
     if (Uout) Uout += vox0 * C;
 
@@ -745,16 +748,17 @@ __global__ void k_fill_load(Geo g, Ptr<T> p, int buf)
     }
 }
 
-// FP32 -> FP64 state transfer at the precision switch: U64[it&1] <- U32[it&1]
+// FP32 -> FP64 state transfer at the precision switch: the FP64 iterate
+// `it` <- the FP32 iterate `it` (solves that are done are skipped)
 template <int C>
-__global__ void k_promote(Geo g, const float* U32a, const float* U32b, long long sU32, double* U64a, double* U64b,
-                          long long sU64, const Ctl* ctl)
+__global__ void k_promote(Geo g, Ptr<float> p32, Ptr<double> p64, const Ctl* ctl)
 {
     const long long n = g.N * C;
     for (int s = 0; s < g.nsolve; ++s) {
-        const int b = ctl[s].it & 1;
-        const float* src = (b ? U32b : U32a) + (long long)s * sU32;
-        double* dst = (b ? U64b : U64a) + (long long)s * sU64;
+        if (ctl[s].done) continue;
+        const int it = ctl[s].it;
+        const float* src = u_buf(p32, it, s);
+        double* dst = u_buf(p64, it, s);
         for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
             dst[i] = (double)src[i];
     }
@@ -768,8 +772,7 @@ __global__ void __launch_bounds__(256) k_means(Geo g, Ptr<T> p, double* out, int
     __shared__ double red[256 / 64];
     constexpr int C = PH::C;
     const int s = blockIdx.y;
-    const int b = p.ctl[s].it & 1;
-    const T* U = u_buf(p, b, s);
+    const T* U = u_buf(p, p.ctl[s].it, s);
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh;
     const T* coef = p.coef + (long long)rv * p.sCoef;
@@ -797,8 +800,7 @@ template <typename T, typename PH>
 __global__ void k_fields(Geo g, Ptr<T> p, int s, double* uo, double* vo)
 {
     constexpr int C = PH::C;
-    const int b = p.ctl[s].it & 1;
-    const T* U = u_buf(p, b, s);
+    const T* U = u_buf(p, p.ctl[s].it, s);
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh;
     const T* coef = p.coef + (long long)rv * p.sCoef;
