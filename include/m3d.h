@@ -59,8 +59,8 @@ typedef struct {
 
 /* FFT_Analysis3D(int Nx, int Ny, int Nz, string mater_behav)
  * -- include/FFT_analysis3D.hpp:196, src/3d/FFT_analysis3D.cpp:25-152.
- * behav: "e" (accelerated scheme); "ep"/"ed" are declared by the reference
- * but not provided by this library yet (M3D_ERR_UNSUPPORTED). */
+ * behav: "e" (accelerated scheme), "ep" (J2 elastoplastic) or "ed" (damage);
+ * "ep"/"ed" runAnalysis runs the basic scheme in FP64 whatever `precision`. */
 int m3d_elastic_create(int nx, int ny, int nz, const char *behav, int precision, m3d_elastic **out);
 
 /* ~FFT_Analysis3D() -- src/3d/FFT_analysis3D.cpp:1107-1147 */
@@ -77,8 +77,17 @@ int m3d_elastic_set_load(m3d_elastic *h, const double *E, const double *X0);
 
 /* void runAnalysis(double tol_field, double tol_eq, int nInc)
  * -- include/FFT_analysis3D.hpp:206, src/3d/FFT_analysis3D.cpp:749-854.
- * info may be NULL. */
+ * "e": accelerated scheme to the stop rule (nInc ignored, as in the
+ * reference).  "ep"/"ed": nInc increments of the basic scheme with mixed
+ * control along X0 (updateE); info (may be NULL) describes the last increment,
+ * iters_fp64 the total. */
 int m3d_elastic_run(m3d_elastic *h, double tol_field, double tol_eq, int nInc, m3d_run_info *info);
+
+/* public member prop_homogenized -- include/FFT_analysis3D.hpp:156, filled by
+ * export_homogen_var (src/3d/FFT_analysis3D.cpp:722-746) during an "ep"/"ed"
+ * runAnalysis: rows x 9 column-major [eps_t, sigma_1..6, mean sigma_eff (ep)
+ * or d (ed), mean eps_p (ep) or 0].  prop may be NULL to query *rows. */
+int m3d_elastic_get_prop(m3d_elastic *h, double *prop, int *rows);
 
 /* double homogenization(double tol_field, double tol_eq, string out)
  * -- include/FFT_analysis3D.hpp:209, src/3d/FFT_analysis3D.cpp:856-999.
@@ -155,6 +164,47 @@ int m3d_thermal_get_fields(m3d_thermal *h, int lc, double *grad, double *flux);
 int m3d_thermal_set_option(m3d_thermal *h, const char *key, double value);
 
 const char *m3d_thermal_last_error(m3d_thermal *h);
+
+/* ------------------------------------------------------------------ batch */
+
+/* Many independent elastic homogenizations (RVEs of any mix of grid sizes).
+ * The reference has no batch entry point: its callers loop over RVEs,
+ * constructing one FFT_Analysis3D per RVE and calling material() and
+ * homogenization() on each (example/3d/main.cpp:117-128,
+ * src/3d/FFT_analysis3D.cpp:420-549,856-999); results here are those of
+ * that loop.  The library groups the problems by grid shape and runs each
+ * group's 6 x RVEs load cases as one batch of solves on the device. */
+typedef struct m3d_batch m3d_batch;
+
+int m3d_batch_create(int precision, m3d_batch **out);
+
+int m3d_batch_destroy(m3d_batch *b);
+
+/* Adds one RVE: material() of a behaviour-"e" FFT_Analysis3D(nx, ny, nz)
+ * (FFT_analysis3D.cpp:420-549).  mat: nph x 7 column-major as in
+ * m3d_elastic_material; rve: N phase ids, z fastest.  *index receives the
+ * problem's position (0, 1, 2, ...). */
+int m3d_batch_add(m3d_batch *b, int nx, int ny, int nz, const double *mat, const int32_t *rve, int nph,
+                  int *index);
+
+/* homogenization(tol_field, tol_eq, out) of every added RVE
+ * (FFT_analysis3D.cpp:856-999). */
+int m3d_batch_homogenization(m3d_batch *b, double tol_field, double tol_eq);
+
+/* Results of problem `index` after m3d_batch_homogenization, as
+ * m3d_elastic_homogenization returns them (any pointer may be NULL). */
+int m3d_batch_results(m3d_batch *b, int index, double *C, double *S, double *epHomo, double *sigHomo, double *ret,
+                      m3d_run_info *info6);
+
+/* Problem count, device time (ms) of the last m3d_batch_homogenization and
+ * its voxel-iterations (sum over solves of N * iterations). */
+int m3d_batch_stats(m3d_batch *b, int *count, double *ms, double *voxel_iterations);
+
+/* options as m3d_elastic_set_option, plus "max_solves": upper bound on the
+ * solves of one device batch (0 = as many as fit in 70% of free memory). */
+int m3d_batch_set_option(m3d_batch *b, const char *key, double value);
+
+const char *m3d_batch_last_error(m3d_batch *b);
 
 /* library version string */
 const char *m3d_version(void);

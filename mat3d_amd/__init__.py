@@ -34,10 +34,12 @@ EXPORTS = [
     "m3d_version",
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
-    "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile",
+    "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
+    "m3d_batch_create", "m3d_batch_destroy", "m3d_batch_add", "m3d_batch_homogenization", "m3d_batch_results",
+    "m3d_batch_stats", "m3d_batch_set_option", "m3d_batch_last_error",
 ]
 
 
@@ -84,6 +86,7 @@ def lib():
             "m3d_elastic_set_option": (ip, [vp, cp, ct.c_double]),
             "m3d_elastic_last_error": (cp, [vp]),
             "m3d_elastic_profile": (ip, [vp, ip, ip, dp, ipp]),
+            "m3d_elastic_get_prop": (ip, [vp, dp, ipp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -94,6 +97,14 @@ def lib():
             "m3d_thermal_get_fields": (ip, [vp, ip, dp, dp]),
             "m3d_thermal_set_option": (ip, [vp, cp, ct.c_double]),
             "m3d_thermal_last_error": (cp, [vp]),
+            "m3d_batch_create": (ip, [ip, ct.POINTER(vp)]),
+            "m3d_batch_destroy": (ip, [vp]),
+            "m3d_batch_add": (ip, [vp, ip, ip, ip, dp, i32p, ip, ipp]),
+            "m3d_batch_homogenization": (ip, [vp, ct.c_double, ct.c_double]),
+            "m3d_batch_results": (ip, [vp, ip, dp, dp, dp, dp, dp, rip]),
+            "m3d_batch_stats": (ip, [vp, ipp, dp, dp]),
+            "m3d_batch_set_option": (ip, [vp, cp, ct.c_double]),
+            "m3d_batch_last_error": (cp, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -194,6 +205,15 @@ class FFT_Analysis3D:
         self._chk(lib().m3d_elastic_get_fields(self._h, int(lc), _dp(eps), _dp(sig)))
         return eps, sig
 
+    @property
+    def prop_homogenized(self):
+        """rows x 9 [eps_t, sigma_1..6, sigma_eff|d, eps_p|0] of the last "ep"/"ed" run."""
+        rows = ct.c_int(0)
+        self._chk(lib().m3d_elastic_get_prop(self._h, None, ct.byref(rows)))
+        out = np.zeros(9 * max(rows.value, 1))
+        self._chk(lib().m3d_elastic_get_prop(self._h, _dp(out), ct.byref(rows)))
+        return out[: 9 * rows.value].reshape(9, rows.value).T.copy()
+
     def profile_kernels(self, precision="fp32_polish", reps=10):
         """Average ms per launch of the per-iteration kernels (HIP events on the
         solver's stream) and the geometry used for byte accounting."""
@@ -283,3 +303,69 @@ class FFT_analysis3d_thermal:
     @property
     def heatFlux(self):
         return self.fields(0)[1]
+
+
+class Batch:
+    """Many independent elastic homogenizations in one call.
+
+    The reference's callers construct one ``FFT_Analysis3D`` per RVE and call
+    ``material`` and ``homogenization`` on each (example/3d/main.cpp:117-128);
+    ``Batch`` returns the results of that loop, with RVEs of one grid shape
+    solved together on the device."""
+
+    def __init__(self, precision="fp32_polish"):
+        h = ct.c_void_p()
+        rc = lib().m3d_batch_create(PRECISION[precision], ct.byref(h))
+        if rc != M3D_OK:
+            raise M3DError(rc, "m3d_batch_create failed")
+        self._h = h
+        self.shapes = []
+
+    def _chk(self, rc):
+        if rc != M3D_OK:
+            raise M3DError(rc, lib().m3d_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().m3d_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_option(self, key, value):
+        self._chk(lib().m3d_batch_set_option(self._h, key.encode(), float(value)))
+
+    def add(self, mat, rve, shape=None):
+        """Adds one RVE (``rve``: (Nx, Ny, Nz) phase ids, or flat with ``shape``);
+        ``mat``: (nph, 7) [E, nu, sigma_y, H, n, P, Y0].  Returns its index."""
+        rve = np.asarray(rve)
+        shape = tuple(rve.shape) if shape is None else tuple(shape)
+        if len(shape) != 3:
+            raise ValueError("shape must be (Nx, Ny, Nz)")
+        flat = np.ascontiguousarray(rve.reshape(-1), dtype=np.int32)
+        if flat.size != shape[0] * shape[1] * shape[2]:
+            raise ValueError("RVE size does not match its shape")
+        mat = np.asarray(mat, dtype=np.float64)
+        m = _colmajor(mat)
+        idx = ct.c_int(-1)
+        self._chk(lib().m3d_batch_add(self._h, shape[0], shape[1], shape[2], _dp(m), _i32p(flat), mat.shape[0],
+                                      ct.byref(idx)))
+        self.shapes.append(shape)
+        return idx.value
+
+    def homogenization(self, tol_field, tol_eq):
+        self._chk(lib().m3d_batch_homogenization(self._h, float(tol_field), float(tol_eq)))
+
+    def result(self, i):
+        C, S, ep, sg = (np.zeros(36) for _ in range(4))
+        ret = ct.c_double(0)
+        infos = (RunInfo * 6)()
+        self._chk(lib().m3d_batch_results(self._h, int(i), _dp(C), _dp(S), _dp(ep), _dp(sg), ct.byref(ret), infos))
+        f = lambda a: a.reshape(6, 6).T.copy()
+        return {"CHomo": f(C), "SHomo": f(S), "epHomo": f(ep), "sigHomo": f(sg), "ret": ret.value,
+                "info": [x.as_dict() for x in infos]}
+
+    def stats(self):
+        n, ms, vi = ct.c_int(0), ct.c_double(0), ct.c_double(0)
+        self._chk(lib().m3d_batch_stats(self._h, ct.byref(n), ct.byref(ms), ct.byref(vi)))
+        return {"count": n.value, "ms": ms.value, "voxel_iterations": vi.value}

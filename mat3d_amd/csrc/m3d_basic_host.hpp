@@ -1,0 +1,269 @@
+// m3d_basic_host.hpp -- host driver of the basic scheme ("ep" / "ed").
+#pragma once
+#include "m3d_basic.hpp"
+#include "m3d_engine.hpp"
+
+namespace {
+
+// ------------------------------------------------------ basic scheme runner
+// runAnalysis "ep" / "ed" (FFT_analysis3D.cpp:796-853) in FP64 on one solve.
+struct BasicRunner {
+    Engine<Elastic>& e;
+    Geo g12{}, g6{};
+    int beh;
+    double* f6 = nullptr;   // eps, sig, sig_old, eps_old (4 x 6N)
+    double* f1 = nullptr;   // ep|d, phi, dep, seff (4 x N)
+    unsigned char* S6 = nullptr;
+    unsigned char* S12 = nullptr;
+    double* psum = nullptr;
+    double* pchk = nullptr;
+    double* bs = nullptr;
+    double* mat = nullptr;
+    Ctl* ctl = nullptr;
+    int nblk = 256;
+    std::vector<double> prop; // rows x 9, column-major like prop_homogenized
+    int rows = 0;
+    m3d_run_info info{};
+
+    BasicRunner(Engine<Elastic>& eng, int b) : e(eng), beh(b)
+    {
+        g12 = make_geo<double>(e.Nx, e.Ny, e.Nz, 12);
+        g6 = make_geo<double>(e.Nx, e.Ny, e.Nz, 6);
+        g12.fz = g12.fy = g12.fx = 0;
+        g6.fz = g6.fy = g6.fx = 0;
+        // make_geo may have switched tiles to the specialised kernels' shapes
+        plan_tiles(g12, sizeof(cx<double>));
+        plan_tiles(g6, sizeof(cx<double>));
+        g12.nsolve = g6.nsolve = 1;
+        const size_t N = (size_t)e.N, H = (size_t)e.H;
+        try {
+            e.alloc(&f6, sizeof(double) * 24 * N);
+            e.alloc(&f1, sizeof(double) * 4 * N);
+            e.alloc(&S6, sizeof(cx<double>) * 12 * H);
+            e.alloc(&S12, sizeof(cx<double>) * 12 * H);
+            e.alloc(&psum, sizeof(double) * NBSUM * g12.nbz);
+            e.alloc(&pchk, sizeof(double) * (g12.nbx + 1));
+            e.alloc(&bs, sizeof(double) * 14 * nblk);
+            e.alloc(&mat, sizeof(double) * 256 * 8);
+            e.alloc(&ctl, sizeof(Ctl));
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    void release()
+    {
+        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)psum, (void*)pchk, (void*)bs, (void*)mat,
+                        (void*)ctl})
+            if (q) (void)hipFree(q);
+        f6 = f1 = psum = pchk = bs = mat = nullptr;
+        S6 = S12 = nullptr;
+        ctl = nullptr;
+    }
+    ~BasicRunner() { release(); }
+
+    BPtr<double> bptr()
+    {
+        BPtr<double> p;
+        std::memset(&p, 0, sizeof p);
+        const long long N = e.N;
+        p.eps = f6;
+        p.sig = f6 + 6 * N;
+        p.sig_old = f6 + 12 * N;
+        p.eps_old = f6 + 18 * N;
+        p.ep = f1;
+        p.phi = f1 + N;
+        p.dep = f1 + 2 * N;
+        p.seff = f1 + 3 * N;
+        p.sU = 6 * N;
+        p.sN = N;
+        p.S6 = reinterpret_cast<cx<double>*>(S6);
+        p.S12 = reinterpret_cast<cx<double>*>(S12);
+        p.sS6 = p.sS12 = 12 * e.H;
+        p.ph = e.ph;
+        p.mat = mat;
+        p.twz = e.tw64[0];
+        p.twy = e.tw64[1];
+        p.twx = e.tw64[2];
+        p.psum = psum;
+        p.pchk = pchk;
+        p.ctl = ctl;
+        return p;
+    }
+    Ptr<double> yptr()
+    {
+        Ptr<double> p;
+        std::memset(&p, 0, sizeof p);
+        p.S = reinterpret_cast<cx<double>*>(S6);
+        p.K = reinterpret_cast<cx<double>*>(S12);
+        p.sS = 12 * e.H;
+        p.twz = e.tw64[0];
+        p.twy = e.tw64[1];
+        p.twx = e.tw64[2];
+        p.ctl = ctl;
+        return p;
+    }
+
+    template <bool GEN>
+    void stage(int mode)
+    {
+        hipStream_t st = e.st;
+        BPtr<double> bp = bptr();
+        Ptr<double> yp = yptr();
+        const size_t zsm = (size_t)g12.TY * 12 * g12.lsz * sizeof(cx<double>);
+        const size_t ysm12 = (size_t)g12.Ny * g12.TKy * 12 * sizeof(cx<double>);
+        const size_t ysm6 = (size_t)g6.Ny * g6.TKy * 6 * sizeof(cx<double>);
+        const size_t xsm = (size_t)g12.Nx * g12.TKx * 12 * sizeof(cx<double>);
+        if (mode == BMODE_START) {
+            if (beh == BEH_EP)
+                hipLaunchKernelGGL((k_zbasic<double, BEH_EP, BMODE_START, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
+            else
+                hipLaunchKernelGGL((k_zbasic<double, BEH_ED, BMODE_START, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
+            hipLaunchKernelGGL((k_bfin1<BMODE_START>), dim3(1), dim3(FTHREADS), 0, st, g12, ctl, psum);
+        } else {
+            if (beh == BEH_EP)
+                hipLaunchKernelGGL((k_zbasic<double, BEH_EP, BMODE_ITER, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
+            else
+                hipLaunchKernelGGL((k_zbasic<double, BEH_ED, BMODE_ITER, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
+            hipLaunchKernelGGL((k_bfin1<BMODE_ITER>), dim3(1), dim3(FTHREADS), 0, st, g12, ctl, psum);
+        }
+        hipLaunchKernelGGL((k_ypass<double, 12, -1, BUF_K, GEN>), dim3(g12.Nx * g12.nky), dim3(XYTHREADS), ysm12, st, g12, yp);
+        hipLaunchKernelGGL((k_xbasic<double, GEN>), dim3(g12.nbx), dim3(XYTHREADS), xsm, st, g12, bp);
+        hipLaunchKernelGGL(k_bfin2, dim3(1), dim3(FTHREADS), 0, st, g12, ctl, pchk);
+        hipLaunchKernelGGL((k_ypass<double, 6, 1, BUF_MAIN, GEN>), dim3(g6.Nx * g6.nky), dim3(XYTHREADS), ysm6, st, g6, yp);
+    }
+
+    void run_stage(int mode)
+    {
+        if (g12.gen) stage<true>(mode);
+        else stage<false>(mode);
+    }
+
+    // mtab: per phase {la, mu, Sy, H, n, P, Y0, 0}; la0/mu0: reference medium;
+    // nc = N (C[0]_00/2 + C[1]_00/2)
+    void run(const std::vector<double>& mtab, double la0, double mu0, double nc, const double* E, const double* X0,
+             double tol_f, double tol_eq, int nInc, int cap)
+    {
+        hipStream_t st = e.st;
+        HCHK(hipMemcpyAsync(mat, mtab.data(), sizeof(double) * mtab.size(), hipMemcpyHostToDevice, st));
+        Ctl c;
+        std::memset(&c, 0, sizeof c);
+        double dEs = 0;
+        for (int i = 0; i < 6; ++i) {
+            c.E[i] = E[i] / (double)nInc; // dE = E/nInc; E = dE (:799)
+            c.X0[i] = X0[i];
+            dEs += c.E[i];
+        }
+        c.dEsum = dEs;
+        c.la0 = la0;
+        c.mu0 = mu0;
+        c.nc = nc;
+        c.cap = cap;
+        c.tol_f = tol_f;
+        c.tol_eq = tol_eq;
+        c.eq_it = -1;
+        HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+        BPtr<double> bp = bptr();
+        hipLaunchKernelGGL((k_binit<double>), dim3(1024), dim3(256), 0, st, g12, bp, 1, beh);
+        int t = 0;
+        for (int i = 0; i < 6; ++i)
+            if (X0[i] != 0) { t = i; break; }
+        prop.assign((size_t)9 * nInc, 0.0);
+        rows = nInc;
+        int nx = 0, total_it = 0;
+        std::vector<double> part((size_t)14 * nblk);
+        const int ck = 4;
+        HCHK(hipEventRecord(e.ev0, st));
+        for (int inc = 0; inc < nInc; ++inc) {
+            // host copy of the device state, then reset the per-increment fields
+            HCHK(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            c.incNo = inc + 1;
+            c.done = 0;
+            c.it = 0;
+            c.err = 0;
+            HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+            run_stage(BMODE_START);
+            HCHK(hipGetLastError());
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < ck; ++k) run_stage(BMODE_ITER);
+            HCHK(hipStreamEndCapture(st, &graph));
+            hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ge != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                throw HipError{ge, "hipGraphInstantiate"};
+            }
+            for (int guard = 0; guard < 100000; ++guard) {
+                hipError_t le = hipGraphLaunch(exec, st);
+                if (le == hipSuccess) le = hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st);
+                if (le == hipSuccess) le = hipStreamSynchronize(st);
+                if (le != hipSuccess) {
+                    (void)hipGraphExecDestroy(exec);
+                    (void)hipGraphDestroy(graph);
+                    throw HipError{le, "basic scheme graph"};
+                }
+                if (c.done) break;
+            }
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (c.done == 3)
+                throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};
+            total_it += c.it;
+            info.iterations = c.it;
+            info.status = c.done;
+            info.eqVar = c.eqvar;
+            info.diff_sum = c.dsum;
+            info.diff_eq = c.deq;
+            // export_homogen_var(inc) (:722-746)
+            if (beh == BEH_EP)
+                hipLaunchKernelGGL((k_bsums<double, BEH_EP>), dim3(nblk, 1), dim3(256), 0, st, g12, bp, bs, nblk);
+            else
+                hipLaunchKernelGGL((k_bsums<double, BEH_ED>), dim3(nblk, 1), dim3(256), 0, st, g12, bp, bs, nblk);
+            HCHK(hipGetLastError());
+            HCHK(hipMemcpyAsync(part.data(), bs, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            double m[14];
+            for (int q = 0; q < 14; ++q) {
+                double a = 0;
+                for (int b = 0; b < nblk; ++b) a += part[(size_t)q * nblk + b];
+                m[q] = a / (double)e.N;
+            }
+            const size_t R = (size_t)nInc;
+            prop[inc + 0 * R] = m[t];
+            for (int j = 1; j < 7; ++j) prop[inc + (size_t)j * R] = m[6 + j - 1];
+            prop[inc + 7 * R] = m[12];
+            prop[inc + 8 * R] = beh == BEH_EP ? m[13] : 0.0;
+            // "ed" softening exit (:829-847), evaluated on the row just computed
+            if (beh == BEH_ED && inc != 0)
+                for (int j = 0; j < 6; ++j)
+                    if (X0[j] != 0 && prop[inc + (size_t)(j + 1) * R] < prop[inc - 1 + (size_t)(j + 1) * R]) nx += 1;
+            if (nx == 3) {
+                rows = inc;
+                break;
+            }
+            // update_old_Variable (:693-719); E += dE does not feed back (updateE recomputes E)
+            if (beh == BEH_EP)
+                hipLaunchKernelGGL((k_bupdate<double, BEH_EP>), dim3(1024), dim3(256), 0, st, g12, bp, 1);
+            else
+                hipLaunchKernelGGL((k_bupdate<double, BEH_ED>), dim3(1024), dim3(256), 0, st, g12, bp, 1);
+        }
+        HCHK(hipEventRecord(e.ev1, st));
+        HCHK(hipStreamSynchronize(st));
+        float ms = 0;
+        HCHK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
+        info.time_ms = ms;
+        info.iters_fp64 = total_it;
+        info.iters_fp32 = 0;
+    }
+
+    void fields(double* eps, double* sig)
+    {
+        const size_t n6 = (size_t)6 * e.N;
+        if (eps) HCHK(hipMemcpy(eps, f6, sizeof(double) * n6, hipMemcpyDeviceToHost));
+        if (sig) HCHK(hipMemcpy(sig, f6 + n6, sizeof(double) * n6, hipMemcpyDeviceToHost));
+    }
+};
+
+} // namespace

@@ -1,28 +1,37 @@
 // m3d_elastic.hip -- C-ABI entry points of the elastic solver (include/m3d.h).
 #include "m3d_engine.hpp"
+#include "m3d_basic_host.hpp"
+#include "m3d_elastic_props.hpp"
 
 // ====================================================================== ABI
 struct m3d_elastic {
     Engine<Elastic>* eng = nullptr;
+    BasicRunner* basic = nullptr; // runAnalysis "ep" / "ed"
+    bool last_basic = false;
     std::string err;
     std::string behav;
-    int nph = 0;
     bool have_mat = false, have_load = false, ran = false;
-    int strain_base = 1;
-    double la0 = 0, mu0 = 0;
-    std::vector<double> vol, Cm, Sm; // per phase 6x6
+    ElasticProps P;
     double E[6] = {0, 0, 0, 0, 0, 0}, X0[6] = {0, 0, 0, 0, 0, 0};
 };
 
-struct m3d_thermal {
-    Engine<Thermal>* eng = nullptr;
+// one RVE of a batch and its results
+struct BatchProblem {
+    int nx, ny, nz;
+    ElasticProps P;
+    std::vector<uint8_t> ids;
+    double C[36], S[36], ep[36], sg[36], ret = 0;
+    m3d_run_info info[6];
+};
+
+struct m3d_batch {
+    int precision = M3D_FP32_POLISH;
     std::string err;
-    int nph = 0;
-    bool have_mat = false, have_load = false, ran = false;
-    int grad_base = 1;
-    double k0 = 0;
-    std::vector<double> k;
-    double E[3] = {0, 0, 0};
+    std::vector<BatchProblem> probs;
+    bool ran = false;
+    double polish_factor = 10.0, fp32_floor = 1e-6;
+    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0;
+    double ms = 0, voxel_its = 0;
 };
 
 namespace {
@@ -47,38 +56,6 @@ int guarded(Hd* h, F&& f)
         h->err = "unexpected error";
         return M3D_ERR_HIP;
     }
-}
-
-// per-phase coefficients of the elastic maps (see m3d_kernels.hpp)
-void elastic_tables(m3d_elastic* h, const std::vector<double>& la, const std::vector<double>& mu,
-                    const std::vector<double>& K, std::vector<double>& coef)
-{
-    const int n = (int)la.size();
-    const double K0 = h->la0 + 2.0 * h->mu0 / 3.0, mu0 = h->mu0;
-    coef.assign((size_t)256 * NCOEF, 0.0);
-    for (int p = 0; p < n; ++p) {
-        double* c = coef.data() + (size_t)p * NCOEF;
-        if (h->strain_base) {
-            c[0] = 3 * K[p];            c[1] = 2 * mu[p];
-            c[2] = 3 * (K[p] - K0);     c[3] = 2 * (mu[p] - mu0);
-            c[4] = 2 * K0 / (K[p] + K0); c[5] = 2 * mu0 / (mu[p] + mu0);
-        } else {
-            c[0] = 1.0 / (3 * K[p]);    c[1] = 1.0 / (2 * mu[p]);
-            c[2] = 1.0 / (3 * K[p]) - 1.0 / (3 * K0);
-            c[3] = 1.0 / (2 * mu[p]) - 1.0 / (2 * mu0);
-            c[4] = 2 * K[p] / (K[p] + K0); c[5] = 2 * mu[p] / (mu[p] + mu0);
-        }
-    }
-}
-
-double elastic_norm(const m3d_elastic* h, const double* E)
-{
-    // N * (0.5 M[0](0,0) + 0.5 M[1](0,0)) * E.sum(), FFT_analysis3D.cpp:399,408
-    const int p1 = h->nph > 1 ? 1 : 0;
-    const std::vector<double>& M = h->strain_base ? h->Cm : h->Sm;
-    double es = 0;
-    for (int i = 0; i < 6; ++i) es += E[i];
-    return (double)h->eng->N * (0.5 * M[0] + 0.5 * M[36 * p1]) * es;
 }
 
 void fill_info(m3d_run_info* dst, const m3d_run_info& src)
@@ -119,6 +96,7 @@ int m3d_elastic_create(int nx, int ny, int nz, const char* behav, int precision,
 int m3d_elastic_destroy(m3d_elastic* h)
 {
     if (!h) return M3D_ERR_ARG;
+    delete h->basic;
     delete h->eng;
     delete h;
     return M3D_OK;
@@ -144,57 +122,31 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
 int m3d_elastic_material(m3d_elastic* h, const double* mat, const int32_t* rve, int nph)
 {
     return guarded(h, [&]() {
-        if (!mat || !rve || nph < 1) throw ArgError{M3D_ERR_ARG, "material: null pointer or nph < 1"};
-        if (nph > 256) throw ArgError{M3D_ERR_UNSUPPORTED, "material: more than 256 phases"};
         Engine<Elastic>& e = *h->eng;
-        const long long N = e.N;
-        std::vector<double> El(nph), nu(nph), mu(nph), la(nph), K(nph);
-        h->Cm.assign((size_t)36 * nph, 0.0);
-        h->Sm.assign((size_t)36 * nph, 0.0);
-        for (int i = 0; i < nph; ++i) {
-            El[i] = mat[i + 0 * nph];
-            nu[i] = mat[i + 1 * nph];
-            mu[i] = El[i] / (1 + nu[i]) / 2.;
-            la[i] = nu[i] * El[i] / (1 + nu[i]) / (1 - 2 * nu[i]);
-            K[i] = la[i] + (2. / 3.) * mu[i];
-            if (!std::isfinite(mu[i]) || !std::isfinite(la[i]) || mu[i] <= 0)
-                throw ArgError{M3D_ERR_NUMERIC, "material: non-finite or non-positive moduli"};
-            iso6(la[i], mu[i], &h->Cm[36 * i]);
-            if (!inv_n(&h->Cm[36 * i], &h->Sm[36 * i], 6)) throw ArgError{M3D_ERR_NUMERIC, "material: singular C"};
-        }
-        std::vector<uint8_t> ids((size_t)N);
-        std::vector<long long> cnt(nph, 0);
-        for (long long i = 0; i < N; ++i) {
-            const int p = rve[i];
-            if (p < 0 || p >= nph) throw ArgError{M3D_ERR_ARG, "material: phase id out of range at voxel " + std::to_string(i)};
-            ids[i] = (uint8_t)p;
-            cnt[p]++;
-        }
-        h->vol.assign(nph, 0.0);
-        for (int k = 0; k < nph; ++k) h->vol[k] = (double)cnt[k] / (double)N;
-        double mu_max, mu_min, la_max, la_min, K_max, K_min, vmax, dummy;
-        int imu, idum;
-        max_min(mu, mu_max, imu, mu_min);
-        max_min(la, la_max, idum, la_min);
-        max_min(K, K_max, idum, K_min);
-        max_min(h->vol, vmax, idum, dummy);
-        if (h->behav == "e") {
-            const double K0 = std::sqrt(K_min * K_max);
-            h->mu0 = std::sqrt(mu_min * mu_max);
-            h->la0 = K0 - h->mu0 * 2. / 3.;
-            h->strain_base = (h->vol[imu] == vmax) ? 1 : 0;
-        } else {
-            h->strain_base = 1;
-            h->mu0 = (mu_min + mu_max) / 2;
-            h->la0 = (la_min + la_max) / 2;
-        }
-        std::vector<double> coef;
-        elastic_tables(h, la, mu, K, coef);
-        std::vector<double> prm = {h->la0, h->mu0};
-        e.upload_material(ids, coef, prm);
-        e.g32.strain_base = e.g64.strain_base = h->strain_base;
-        h->nph = nph;
+        std::vector<uint8_t> ids((size_t)e.N);
+        ElasticProps P;
+        elastic_props(h->behav, mat, rve, nph, e.N, P, ids.data());
+        std::vector<double> coef((size_t)256 * NCOEF);
+        elastic_tables(P, coef.data());
+        e.upload_material(ids, coef, {P.la0, P.mu0});
+        h->P = std::move(P);
         h->have_mat = true;
+        return M3D_OK;
+    });
+}
+
+// public member prop_homogenized (nInc x 9), FFT_analysis3D.hpp:156, filled by
+// export_homogen_var (:722-746)
+int m3d_elastic_get_prop(m3d_elastic* h, double* prop, int* rows)
+{
+    return guarded(h, [&]() {
+        if (!h->basic || !h->last_basic) throw ArgError{M3D_ERR_STATE, "get_prop: no \"ep\"/\"ed\" run"};
+        if (rows) *rows = h->basic->rows;
+        if (prop) {
+            const int R = h->basic->rows, Rall = (int)(h->basic->prop.size() / 9);
+            for (int j = 0; j < 9; ++j)
+                for (int i = 0; i < R; ++i) prop[i + (size_t)j * R] = h->basic->prop[i + (size_t)j * Rall];
+        }
         return M3D_OK;
     });
 }
@@ -203,10 +155,10 @@ int m3d_elastic_info(m3d_elastic* h, double* vol, int* strain_base, double* la0,
 {
     return guarded(h, [&]() {
         if (!h->have_mat) throw ArgError{M3D_ERR_STATE, "info: material() not called"};
-        if (vol) std::memcpy(vol, h->vol.data(), sizeof(double) * h->nph);
-        if (strain_base) *strain_base = h->strain_base;
-        if (la0) *la0 = h->la0;
-        if (mu0) *mu0 = h->mu0;
+        if (vol) std::memcpy(vol, h->P.vol.data(), sizeof(double) * h->P.nph);
+        if (strain_base) *strain_base = h->P.strain_base;
+        if (la0) *la0 = h->P.la0;
+        if (mu0) *mu0 = h->P.mu0;
         return M3D_OK;
     });
 }
@@ -230,12 +182,24 @@ int m3d_elastic_run(m3d_elastic* h, double tol_field, double tol_eq, int nInc, m
 {
     return guarded(h, [&]() {
         if (!h->have_mat || !h->have_load) throw ArgError{M3D_ERR_STATE, "run: material() and setLoad() first"};
-        if (h->behav != "e") throw ArgError{M3D_ERR_UNSUPPORTED, "run: behaviour '" + h->behav + "' not provided yet"};
+        if (h->behav != "e") {
+            // basic scheme with mixed control, J2 ("ep") or damage ("ed"), :796-853
+            if (nInc < 1) throw ArgError{M3D_ERR_ARG, "run: nInc must be >= 1"};
+            if (!h->basic) h->basic = new BasicRunner(*h->eng, h->behav == "ep" ? BEH_EP : BEH_ED);
+            const int p1 = h->P.nph > 1 ? 1 : 0;
+            const double nc = (double)h->eng->N * (0.5 * h->P.Cm[0] + 0.5 * h->P.Cm[36 * p1]);
+            h->basic->run(h->P.mtab, h->P.la0, h->P.mu0, nc, h->E, h->X0, tol_field, tol_eq, nInc, h->eng->cap);
+            fill_info(info, h->basic->info);
+            h->last_basic = true;
+            h->ran = true;
+            return M3D_OK;
+        }
         (void)nInc; // the "e" branch ignores nInc (FFT_analysis3D.cpp:751)
         std::array<double, 6> ld;
         for (int i = 0; i < 6; ++i) ld[i] = h->E[i];
-        h->eng->solve({ld}, tol_field, tol_eq, {elastic_norm(h, h->E)});
+        h->eng->solve({ld}, tol_field, tol_eq, {elastic_norm(h->P, h->eng->N, h->E)}, h->P.strain_base);
         fill_info(info, h->eng->info[0]);
+        h->last_basic = false;
         h->ran = true;
         return M3D_OK;
     });
@@ -248,41 +212,15 @@ int m3d_elastic_homogenization(m3d_elastic* h, double tol_field, double tol_eq, 
     return guarded(h, [&]() {
         if (!h->have_mat) throw ArgError{M3D_ERR_STATE, "homogenization: material() first"};
         Engine<Elastic>& e = *h->eng;
-        std::vector<std::array<double, 6>> loads(6);
-        std::vector<double> norms(6);
-        for (int w = 0; w < 6; ++w) {
-            loads[w].fill(0.0);
-            loads[w][w] = h->strain_base ? 0.0005 : 1.e6; // :872-875
-            norms[w] = elastic_norm(h, loads[w].data());
-        }
-        e.solve(loads, tol_field, tol_eq, norms);
+        std::vector<std::array<double, 6>> loads;
+        std::vector<double> norms;
+        elastic_unit_loads(h->P, e.N, loads, norms);
+        e.solve(loads, tol_field, tol_eq, norms, h->P.strain_base);
         const std::vector<double> m = e.means();
-        double ep[36], sg[36];
-        for (int w = 0; w < 6; ++w)
-            for (int t = 0; t < 6; ++t) {
-                const double mu_ = m[(size_t)w * 12 + t], mv = m[(size_t)w * 12 + 6 + t];
-                const double eps_m = h->strain_base ? mu_ : mv;
-                const double sig_m = h->strain_base ? mv : mu_;
-                sg[w + 6 * t] = sig_m; // sigHomo(w,t)
-                ep[t + 6 * w] = eps_m; // epHomo(t,w)
-            }
-        double Ch[36], Sh[36], inv[36];
-        if (h->strain_base) {
-            if (!inv_n(ep, inv, 6)) throw ArgError{M3D_ERR_NUMERIC, "homogenization: singular epHomo"};
-            mul_n(inv, sg, Ch, 6);
-            if (!inv_n(Ch, Sh, 6)) throw ArgError{M3D_ERR_NUMERIC, "homogenization: singular CHomo"};
-        } else {
-            if (!inv_n(sg, inv, 6)) throw ArgError{M3D_ERR_NUMERIC, "homogenization: singular sigHomo"};
-            mul_n(inv, ep, Sh, 6);
-            if (!inv_n(Sh, Ch, 6)) throw ArgError{M3D_ERR_NUMERIC, "homogenization: singular SHomo"};
-        }
-        if (Cout) std::memcpy(Cout, Ch, sizeof Ch);
-        if (Sout) std::memcpy(Sout, Sh, sizeof Sh);
-        if (epOut) std::memcpy(epOut, ep, sizeof ep);
-        if (sgOut) std::memcpy(sgOut, sg, sizeof sg);
-        if (ret) *ret = 1. / Sh[0] + 1. / Sh[7] + 1. / Sh[14];
+        elastic_assemble(h->P.strain_base, m.data(), Cout, Sout, epOut, sgOut, ret);
         if (info6)
             for (int w = 0; w < 6; ++w) info6[w] = e.info[w];
+        h->last_basic = false;
         h->ran = true;
         return M3D_OK;
     });
@@ -292,8 +230,13 @@ int m3d_elastic_get_fields(m3d_elastic* h, int lc, double* eps, double* sig)
 {
     return guarded(h, [&]() {
         if (!h->ran) throw ArgError{M3D_ERR_STATE, "get_fields: nothing has run"};
+        if (h->last_basic) {
+            if (lc != 0) throw ArgError{M3D_ERR_ARG, "get_fields: runAnalysis holds one load case"};
+            h->basic->fields(eps, sig);
+            return M3D_OK;
+        }
         // u is epsilon in the strain base and sigma in the stress base
-        if (h->strain_base) h->eng->fields(lc, eps, sig);
+        if (h->P.strain_base) h->eng->fields(lc, eps, sig);
         else h->eng->fields(lc, sig, eps);
         return M3D_OK;
     });
@@ -306,6 +249,163 @@ int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, in
         h->eng->profile(precision, reps, ms6);
         if (geom9) h->eng->geometry(precision, geom9);
         h->ran = false; // the fields were overwritten
+        return M3D_OK;
+    });
+}
+
+// ------------------------------------------------------------------ batch
+int m3d_batch_create(int precision, m3d_batch** out)
+{
+    if (!out) return M3D_ERR_ARG;
+    *out = nullptr;
+    if (precision != M3D_FP64 && precision != M3D_FP32_POLISH) return M3D_ERR_ARG;
+    m3d_batch* b = new (std::nothrow) m3d_batch;
+    if (!b) return M3D_ERR_OOM;
+    b->precision = precision;
+    *out = b;
+    return M3D_OK;
+}
+
+int m3d_batch_destroy(m3d_batch* b)
+{
+    if (!b) return M3D_ERR_ARG;
+    delete b;
+    return M3D_OK;
+}
+
+const char* m3d_batch_last_error(m3d_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+int m3d_batch_set_option(m3d_batch* b, const char* key, double value)
+{
+    return guarded(b, [&]() {
+        std::string k = key ? key : "";
+        if (k == "polish_factor" && value >= 1.0) b->polish_factor = value;
+        else if (k == "fp32_floor" && value >= 0.0) b->fp32_floor = value;
+        else if (k == "rollback" && value >= 0 && value <= 64) b->rollback = (int)value;
+        else if (k == "cap" && value >= 1) b->cap = (int)value;
+        else if (k == "chunk" && value >= 0) b->chunk = (int)value;
+        else if (k == "max_solves" && value >= 0) b->max_solves = (int)value;
+        else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
+        return M3D_OK;
+    });
+}
+
+// material() of one more RVE, FFT_analysis3D.cpp:420-549
+int m3d_batch_add(m3d_batch* b, int nx, int ny, int nz, const double* mat, const int32_t* rve, int nph, int* index)
+{
+    return guarded(b, [&]() {
+        if (nx < 2 || ny < 2 || nz < 2 || (long long)nx * ny * nz > (1LL << 31))
+            throw ArgError{M3D_ERR_ARG, "batch_add: grid sizes"};
+        (void)make_geo<double>(nx, ny, nz, 6); // throws for unsupported sizes
+        (void)make_geo<float>(nx, ny, nz, 6);
+        BatchProblem q;
+        q.nx = nx;
+        q.ny = ny;
+        q.nz = nz;
+        const long long N = (long long)nx * ny * nz;
+        q.ids.resize((size_t)N);
+        elastic_props("e", mat, rve, nph, N, q.P, q.ids.data());
+        b->probs.push_back(std::move(q));
+        b->ran = false;
+        if (index) *index = (int)b->probs.size() - 1;
+        return M3D_OK;
+    });
+}
+
+// homogenization() of every RVE, FFT_analysis3D.cpp:856-999: problems of one
+// grid shape share an engine; their 6 load cases each run as one batch of
+// independent solves (sub-batched to fit the device memory)
+int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
+{
+    return guarded(b, [&]() {
+        if (b->probs.empty()) throw ArgError{M3D_ERR_STATE, "batch_homogenization: no problems added"};
+        b->ran = false;
+        b->ms = 0;
+        b->voxel_its = 0;
+        std::vector<std::array<int, 3>> shapes;
+        for (const BatchProblem& q : b->probs) {
+            std::array<int, 3> sh{q.nx, q.ny, q.nz};
+            if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
+        }
+        for (const auto& sh : shapes) {
+            std::vector<int> members;
+            for (int i = 0; i < (int)b->probs.size(); ++i)
+                if (b->probs[i].nx == sh[0] && b->probs[i].ny == sh[1] && b->probs[i].nz == sh[2]) members.push_back(i);
+            Engine<Elastic> e(sh[0], sh[1], sh[2], b->precision);
+            e.polish_factor = b->polish_factor;
+            e.fp32_floor = b->fp32_floor;
+            e.rollback = b->rollback;
+            e.cap = b->cap;
+            e.chunk = b->chunk;
+            // RVEs per device batch: 70% of free memory, or the max_solves option
+            size_t fr = 0, tot = 0;
+            HCHK(hipMemGetInfo(&fr, &tot));
+            long long per = (long long)(0.7 * (double)fr / (double)(6 * e.bytes_per_solve() + (size_t)e.N));
+            if (b->max_solves > 0) per = std::min<long long>(per, b->max_solves / 6);
+            if (per < 1) throw ArgError{M3D_ERR_OOM, "batch_homogenization: one RVE's 6 solves do not fit"};
+            for (size_t k0 = 0; k0 < members.size(); k0 += (size_t)per) {
+                const size_t k1 = std::min(members.size(), k0 + (size_t)per);
+                const int R = (int)(k1 - k0);
+                std::vector<uint8_t> ids((size_t)R * e.N);
+                std::vector<double> coef((size_t)R * 256 * NCOEF), prm((size_t)2 * R);
+                std::vector<std::array<double, 6>> loads;
+                std::vector<double> norms;
+                std::vector<int> sbv, rvv;
+                for (int r = 0; r < R; ++r) {
+                    const BatchProblem& q = b->probs[members[k0 + r]];
+                    std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
+                    elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF);
+                    prm[2 * r] = q.P.la0;
+                    prm[2 * r + 1] = q.P.mu0;
+                    elastic_unit_loads(q.P, e.N, loads, norms);
+                    for (int w = 0; w < 6; ++w) {
+                        sbv.push_back(q.P.strain_base);
+                        rvv.push_back(r);
+                    }
+                }
+                e.upload_material(ids, coef, prm);
+                e.solve(loads, tol_field, tol_eq, norms, 1, sbv, rvv);
+                const std::vector<double> m = e.means();
+                for (int r = 0; r < R; ++r) {
+                    BatchProblem& q = b->probs[members[k0 + r]];
+                    elastic_assemble(q.P.strain_base, m.data() + (size_t)r * 72, q.C, q.S, q.ep, q.sg, &q.ret);
+                    for (int w = 0; w < 6; ++w) {
+                        q.info[w] = e.info[(size_t)6 * r + w];
+                        b->voxel_its += (double)e.N * q.info[w].iterations;
+                    }
+                }
+                b->ms += e.info.empty() ? 0.0 : e.info[0].time_ms;
+            }
+        }
+        b->ran = true;
+        return M3D_OK;
+    });
+}
+
+int m3d_batch_results(m3d_batch* b, int index, double* C, double* S, double* epHomo, double* sigHomo, double* ret,
+                      m3d_run_info* info6)
+{
+    return guarded(b, [&]() {
+        if (!b->ran) throw ArgError{M3D_ERR_STATE, "batch_results: run m3d_batch_homogenization first"};
+        if (index < 0 || index >= (int)b->probs.size()) throw ArgError{M3D_ERR_ARG, "batch_results: index out of range"};
+        const BatchProblem& q = b->probs[index];
+        if (C) std::memcpy(C, q.C, sizeof q.C);
+        if (S) std::memcpy(S, q.S, sizeof q.S);
+        if (epHomo) std::memcpy(epHomo, q.ep, sizeof q.ep);
+        if (sigHomo) std::memcpy(sigHomo, q.sg, sizeof q.sg);
+        if (ret) *ret = q.ret;
+        if (info6)
+            for (int w = 0; w < 6; ++w) info6[w] = q.info[w];
+        return M3D_OK;
+    });
+}
+
+int m3d_batch_stats(m3d_batch* b, int* count, double* ms, double* voxel_iterations)
+{
+    return guarded(b, [&]() {
+        if (count) *count = (int)b->probs.size();
+        if (ms) *ms = b->ms;
+        if (voxel_iterations) *voxel_iterations = b->voxel_its;
         return M3D_OK;
     });
 }

@@ -9,6 +9,7 @@
 // replays.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -240,6 +241,7 @@ struct Engine {
     double* mpart = nullptr;
     int* rve_of = nullptr;
     uint8_t* ph = nullptr;
+    size_t ph_cap = 0;
     float* coef32 = nullptr;
     double* coef64 = nullptr;
     float* prm32 = nullptr;
@@ -281,6 +283,7 @@ struct Engine {
             HCHK(hipMemcpy(tw64[a], t64.data(), sizeof(cx<double>) * lens[a], hipMemcpyHostToDevice));
         }
         HCHK(hipMalloc(&ph, (size_t)N));
+        ph_cap = (size_t)N;
     }
 
     void free_solves()
@@ -335,6 +338,15 @@ struct Engine {
 
     int ring_needed() const { return std::max(2, rollback + 1); }
 
+    // device bytes one solve adds (fields, spectra), for batch sizing
+    size_t bytes_per_solve() const
+    {
+        const size_t nc = (size_t)N * C, hc = (size_t)H * C;
+        size_t b = sizeof(double) * nc * 2 + 2 * sizeof(cx<double>) * hc;
+        if (precision == M3D_FP32_POLISH) b += sizeof(float) * nc * ring_needed();
+        return b;
+    }
+
     void ensure_solves(int n)
     {
         if (n <= cap_solves && (precision != M3D_FP32_POLISH || ring32 == ring_needed())) return;
@@ -388,7 +400,7 @@ struct Engine {
         p.ph = ph;
         p.sPh = N;
         p.rve_of = rve_of;
-        p.sCoef = 0;
+        p.sCoef = 256 * NCOEF;
         p.psum = psum;
         p.pchk = pchk;
         p.ctl = ctl;
@@ -579,10 +591,16 @@ struct Engine {
     }
 
     // loads: nsolve x 6 (only the first C entries used); tol per phase
-    void solve(const std::vector<std::array<double, 6>>& loads, double tol_f, double tol_eq, const std::vector<double>& norm)
+    // loads, norms, base flags and RVE indices per solve (sb/rve empty: the
+    // handle's single RVE with base flag sb0)
+    void solve(const std::vector<std::array<double, 6>>& loads, double tol_f, double tol_eq, const std::vector<double>& norm,
+               int sb0, const std::vector<int>& sbv = {}, const std::vector<int>& rve = {})
     {
         nsolve = (int)loads.size();
         ensure_solves(nsolve);
+        std::vector<int> rv(nsolve, 0);
+        for (int s = 0; s < nsolve && s < (int)rve.size(); ++s) rv[s] = rve[s];
+        HCHK(hipMemcpyAsync(rve_of, rv.data(), sizeof(int) * nsolve, hipMemcpyHostToDevice, st));
         std::vector<Ctl> hctl(nsolve);
         for (int s = 0; s < nsolve; ++s) {
             Ctl& c = hctl[s];
@@ -590,6 +608,7 @@ struct Engine {
             c.eq_it = -1;
             c.cap = cap;
             c.norm = norm[s];
+            c.sb = s < (int)sbv.size() ? sbv[s] : sb0;
             for (int k = 0; k < 6; ++k) c.E[k] = loads[s][k];
             // the FP32 phase stops at polish_factor x tol, but never below what
             // FP32 fields resolve (fp32_floor); the FP64 polish does the rest
@@ -772,9 +791,17 @@ struct Engine {
     }
 
     // upload phase ids (validated, < 256 phases) and coefficient tables
+    // ids: N phase ids per RVE; coef: 256*NCOEF per RVE; prmv: 2 per RVE
     void upload_material(const std::vector<uint8_t>& ids, const std::vector<double>& coef, const std::vector<double>& prmv)
     {
-        HCHK(hipMemcpy(ph, ids.data(), (size_t)N, hipMemcpyHostToDevice));
+        if (ids.size() > ph_cap) {
+            if (ph) (void)hipFree(ph);
+            ph = nullptr;
+            ph_cap = 0;
+            alloc(&ph, ids.size());
+            ph_cap = ids.size();
+        }
+        HCHK(hipMemcpy(ph, ids.data(), ids.size(), hipMemcpyHostToDevice));
         std::vector<float> c32(coef.begin(), coef.end());
         std::vector<float> p32(prmv.begin(), prmv.end());
         if (coef32) hipFree(coef32);

@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
                 for (int c = 0; c < C; ++c) acc += (double)o0[c] + (double)o1[c];
                 PH::polar(k0, u0, o0);
                 PH::polar(k1, u1, o1);
-            } else if (g.strain_base) {
+            } else if (ctl.sb) {
                 PH::dual(k0, u0, o0);
                 PH::dual(k1, u1, o1);
             } else {
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(XYNT) kxy(Geo g, Ptr<T> p, const T* __restrict
 #pragma unroll
                 for (int c = 0; c < C; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
             } else {
-                green_apply<T, PH>(g, prm, kx, y, kz, v);
+                green_apply<T, PH>(g, ctl.sb, prm, kx, y, kz, v);
             }
         }
         __syncthreads();

@@ -37,7 +37,7 @@ constexpr int XYTHREADS = 512;
 constexpr int EPT = 16;  // complex values per thread per pass
 
 enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3 };
-enum { BUF_MAIN = 0, BUF_CHK_CUR = 2, BUF_CHK_PREV = 3 };
+enum { BUF_MAIN = 0, BUF_CHK_CUR = 2, BUF_CHK_PREV = 3, BUF_K = 4 };
 
 // per-solve control block, read and written on the device only
 struct Ctl {
@@ -47,7 +47,9 @@ struct Ctl {
     int need_prev; // ... and of the previous iterate
     int eq_it;     // iterate index eq_val belongs to (-1: none)
     int cap;       // iteration cap (reference: 2000 / 500)
-    int first;     // 1 until the first check of this solve ran
+    int first;     // basic scheme: 1 while the stage after the increment start runs
+    int err;       // basic scheme: constitutive update failed (Newton NaN / no convergence)
+    int sb;        // elastic: 1 strain base, 0 stress base; thermal: 1 gradient base
     int pad;
     double sum_prev;  // sum of the dual field at iterate `it`
     double eq_val;    // eqVar at iterate eq_it
@@ -55,6 +57,12 @@ struct Ctl {
     double tol_f, tol_eq;
     double norm;      // N * (M[0]_00/2 + M[1]_00/2) * sum(E)
     double E[6];      // macroscopic load (DC of the Green output)
+    // basic scheme with mixed control (updateE, FFT_analysis3D.cpp:583-594)
+    double X0[6];     // stress direction
+    double dEsum;     // dE.sum()
+    double incNo;     // increment number (1-based)
+    double nc;        // N * (C[0]_00/2 + C[1]_00/2): Delta-sum normalisation without sum(E)
+    double la0, mu0;  // reference medium (S0 = C0^-1 applied in closed form)
 };
 
 struct Geo {
@@ -69,7 +77,6 @@ struct Geo {
     int nsolve;
     int nbz, nbx;       // blocks per solve of the z pass and the x pass
     double invN;
-    int strain_base;    // elastic: 1 strain base, 0 stress base; thermal: 1 gradient base
     int fz, fy, fx;     // host only: compile-time specialised kernel used for this axis
     int gen;            // host only: generic kernels need the odd radices
 };
@@ -386,7 +393,7 @@ __global__ void __launch_bounds__(ZTHREADS) k_zpass(Geo g, Ptr<T> p)
             PH::polar(kp, u, o);
         } else {
             // sigma: the dual field in the strain (gradient) base, u itself otherwise
-            if (g.strain_base) PH::dual(kp, u, o);
+            if (ctl.sb) PH::dual(kp, u, o);
             else {
 #pragma unroll
                 for (int c = 0; c < C; ++c) o[c] = u[c];
@@ -499,9 +506,9 @@ __device__ __forceinline__ void freq_el(const Geo& g, int kx, int ky, int kz, in
 // thermal maps: gradient base and check use the N/2-positive map
 // (FFT_thermal3D.cpp:113-120); flux base the (N-1)/2 map with -(Ny-x) for x (:150-151)
 __device__ __forceinline__ int fmap_t(int k, int N) { return k <= N / 2 ? k : -(N - k); }
-__device__ __forceinline__ void freq_th(const Geo& g, int kx, int ky, int kz, int f[3])
+__device__ __forceinline__ void freq_th(const Geo& g, int sb, int kx, int ky, int kz, int f[3])
 {
-    if (g.strain_base) {
+    if (sb) {
         f[0] = fmap_t(kz, g.Nz);
         f[1] = fmap_t(ky, g.Ny);
         f[2] = fmap_t(kx, g.Nx);
@@ -513,18 +520,18 @@ __device__ __forceinline__ void freq_th(const Geo& g, int kx, int ky, int kz, in
 }
 
 template <typename T, typename PH>
-__device__ __forceinline__ void green_apply(const Geo& g, const T* prm, int kx, int ky, int kz, cx<T>* v)
+__device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, int kx, int ky, int kz, cx<T>* v)
 {
     constexpr int C = PH::C;
     int f[3];
     if constexpr (C == 6) freq_el(g, kx, ky, kz, f);
-    else freq_th(g, kx, ky, kz, f);
+    else freq_th(g, sb, kx, ky, kz, f);
     T xi[3] = {T(f[0]), T(f[1]), T(f[2])};
     cx<T> in[C], o[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) in[c] = v[c];
-    if constexpr (C == 6) green_elastic<T>(xi, in, o, g.strain_base, prm[0], prm[1]);
-    else green_thermal<T>(xi, in, o, g.strain_base, prm[0]);
+    if constexpr (C == 6) green_elastic<T>(xi, in, o, sb, prm[0], prm[1]);
+    else green_thermal<T>(xi, in, o, sb, prm[0]);
     // c2r of a multi-dimensional half spectrum keeps only the Hermitian part
     // of the kz=0 and kz=Nz/2 planes: average with the operator of the
     // conjugate-partner index when its frequency vector is not -xi.
@@ -533,12 +540,12 @@ __device__ __forceinline__ void green_apply(const Geo& g, const T* prm, int kx, 
         const int px = (g.Nx - kx) % g.Nx, py = (g.Ny - ky) % g.Ny;
         int fp[3];
         if constexpr (C == 6) freq_el(g, px, py, kz, fp);
-        else freq_th(g, px, py, kz, fp);
+        else freq_th(g, sb, px, py, kz, fp);
         if (fp[0] != -f[0] || fp[1] != -f[1] || fp[2] != -f[2]) {
             T xp[3] = {T(fp[0]), T(fp[1]), T(fp[2])};
             cx<T> o2[C];
-            if constexpr (C == 6) green_elastic<T>(xp, in, o2, g.strain_base, prm[0], prm[1]);
-            else green_thermal<T>(xp, in, o2, g.strain_base, prm[0]);
+            if constexpr (C == 6) green_elastic<T>(xp, in, o2, sb, prm[0], prm[1]);
+            else green_thermal<T>(xp, in, o2, sb, prm[0]);
 #pragma unroll
             for (int c = 0; c < C; ++c) o[c] = cscale(o[c] + o2[c], T(0.5));
         }
@@ -583,7 +590,7 @@ __global__ void __launch_bounds__(XYTHREADS) k_xgreen(Geo g, Ptr<T> p, const T* 
 #pragma unroll
             for (int c = 0; c < C; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
         } else {
-            green_apply<T, PH>(g, prm, kx, y, kz, v);
+            green_apply<T, PH>(g, ctl.sb, prm, kx, y, kz, v);
         }
     }
     __syncthreads();

@@ -2,18 +2,6 @@
 #include "m3d_engine.hpp"
 
 // ====================================================================== ABI
-struct m3d_elastic {
-    Engine<Elastic>* eng = nullptr;
-    std::string err;
-    std::string behav;
-    int nph = 0;
-    bool have_mat = false, have_load = false, ran = false;
-    int strain_base = 1;
-    double la0 = 0, mu0 = 0;
-    std::vector<double> vol, Cm, Sm; // per phase 6x6
-    double E[6] = {0, 0, 0, 0, 0, 0}, X0[6] = {0, 0, 0, 0, 0, 0};
-};
-
 struct m3d_thermal {
     Engine<Thermal>* eng = nullptr;
     std::string err;
@@ -47,38 +35,6 @@ int guarded(Hd* h, F&& f)
         h->err = "unexpected error";
         return M3D_ERR_HIP;
     }
-}
-
-// per-phase coefficients of the elastic maps (see m3d_kernels.hpp)
-void elastic_tables(m3d_elastic* h, const std::vector<double>& la, const std::vector<double>& mu,
-                    const std::vector<double>& K, std::vector<double>& coef)
-{
-    const int n = (int)la.size();
-    const double K0 = h->la0 + 2.0 * h->mu0 / 3.0, mu0 = h->mu0;
-    coef.assign((size_t)256 * NCOEF, 0.0);
-    for (int p = 0; p < n; ++p) {
-        double* c = coef.data() + (size_t)p * NCOEF;
-        if (h->strain_base) {
-            c[0] = 3 * K[p];            c[1] = 2 * mu[p];
-            c[2] = 3 * (K[p] - K0);     c[3] = 2 * (mu[p] - mu0);
-            c[4] = 2 * K0 / (K[p] + K0); c[5] = 2 * mu0 / (mu[p] + mu0);
-        } else {
-            c[0] = 1.0 / (3 * K[p]);    c[1] = 1.0 / (2 * mu[p]);
-            c[2] = 1.0 / (3 * K[p]) - 1.0 / (3 * K0);
-            c[3] = 1.0 / (2 * mu[p]) - 1.0 / (2 * mu0);
-            c[4] = 2 * K[p] / (K[p] + K0); c[5] = 2 * mu[p] / (mu[p] + mu0);
-        }
-    }
-}
-
-double elastic_norm(const m3d_elastic* h, const double* E)
-{
-    // N * (0.5 M[0](0,0) + 0.5 M[1](0,0)) * E.sum(), FFT_analysis3D.cpp:399,408
-    const int p1 = h->nph > 1 ? 1 : 0;
-    const std::vector<double>& M = h->strain_base ? h->Cm : h->Sm;
-    double es = 0;
-    for (int i = 0; i < 6; ++i) es += E[i];
-    return (double)h->eng->N * (0.5 * M[0] + 0.5 * M[36 * p1]) * es;
 }
 
 void fill_info(m3d_run_info* dst, const m3d_run_info& src)
@@ -192,7 +148,6 @@ int thermal_material_impl(m3d_thermal* h, const std::vector<double>& k, const in
     }
     std::vector<double> prm = {h->k0, 0.0};
     e.upload_material(ids, coef, prm);
-    e.g32.strain_base = e.g64.strain_base = h->grad_base;
     h->k = k;
     h->nph = nph;
     h->have_mat = true;
@@ -242,7 +197,7 @@ int m3d_thermal_run(m3d_thermal* h, double tol_field, double tol_eq, m3d_run_inf
         if (!h->have_mat || !h->have_load) throw ArgError{M3D_ERR_STATE, "run: material() and setLoad() first"};
         std::array<double, 6> ld{};
         for (int i = 0; i < 3; ++i) ld[i] = h->E[i];
-        h->eng->solve({ld}, tol_field, tol_eq, {thermal_norm(h, h->E)});
+        h->eng->solve({ld}, tol_field, tol_eq, {thermal_norm(h, h->E)}, h->grad_base);
         fill_info(info, h->eng->info[0]);
         h->ran = true;
         return M3D_OK;
@@ -262,7 +217,7 @@ int m3d_thermal_homogenization(m3d_thermal* h, double tol_field, double tol_eq, 
             loads[w][w] = 1; // :506,548
             norms[w] = thermal_norm(h, loads[w].data());
         }
-        e.solve(loads, tol_field, tol_eq, norms);
+        e.solve(loads, tol_field, tol_eq, norms, h->grad_base);
         const std::vector<double> m = e.means();
         double G[9], Q[9], Kh[9], Rh[9], inv[9];
         for (int w = 0; w < 3; ++w)

@@ -125,18 +125,20 @@ def fibres(n, radius, frac, seed, length=64, incl=0, matrix=1):
 
 
 def voronoi(n, grains, seed):
-    """Periodic Voronoi polycrystal: phase id = index of the nearest seed
-    (minimum image).  Uses scipy's periodic k-d tree."""
+    """Periodic Voronoi polycrystal on an n^3 grid (or an (Nx, Ny, Nz) grid
+    when n is a tuple): phase id = index of the nearest seed (minimum image).
+    Uses scipy's periodic k-d tree."""
     from scipy.spatial import cKDTree
 
+    shape = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
     rng = np.random.default_rng(seed)
-    pts = rng.uniform(0, n, size=(grains, 3))
-    tree = cKDTree(pts, boxsize=n)
-    g = np.arange(n) + 0.0
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) % n
+    box = np.array(shape, dtype=np.float64)
+    pts = rng.uniform(0, 1, size=(grains, 3)) * box
+    tree = cKDTree(pts, boxsize=box)
+    X, Y, Z = np.meshgrid(*(np.arange(v) + 0.0 for v in shape), indexing="ij")
+    q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) % box
     _, idx = tree.query(q, k=1)
-    return idx.reshape(n, n, n).astype(np.int32)
+    return idx.reshape(shape).astype(np.int32)
 
 
 # ---------------------------------------------------------------- workloads

@@ -4,6 +4,8 @@ import ctypes as ct
 import os
 import re
 
+import pytest
+
 import mat3d_amd as m
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -38,3 +40,32 @@ def test_null_handle_is_an_error_not_a_crash():
     assert L.m3d_elastic_create(8, 8, 8, b"xx", 0, ct.byref(out)) == -1
     assert L.m3d_elastic_create(8, 8, 8, b"e", 7, ct.byref(out)) == -1
     assert L.m3d_thermal_create(8, 8, 0, 0, ct.byref(out)) == -1
+
+
+def test_batch_host_side_validation():
+    """m3d_batch_create/add only touch the host: argument errors come back as codes."""
+    import numpy as np
+
+    from mat3d_amd import gen
+
+    b = m.Batch("fp64")
+    with pytest.raises(m.M3DError) as e:
+        b.homogenization(1e-6, 1e-6)  # nothing added
+    assert e.value.code == -5
+    with pytest.raises(m.M3DError) as e:
+        b.add(gen.MAT_C1, np.full((8, 8, 8), 5, dtype=np.int32))  # phase id out of range
+    assert e.value.code == -1
+    with pytest.raises(m.M3DError) as e:
+        b.add(gen.MAT_C1, np.zeros((17, 17, 17), dtype=np.int32))  # prime factor 17
+    assert e.value.code == -2
+    with pytest.raises(m.M3DError):
+        b.result(0)
+    assert b.add(gen.MAT_C1, gen.spheres(8, 2, 0.1, seed=1)) == 0
+    assert b.add(gen.MAT_C1, np.ones((6, 8, 10), dtype=np.int32)) == 1
+    assert b.stats()["count"] == 2
+    with pytest.raises(m.M3DError):
+        b.set_option("no_such_option", 1)
+    b.close()
+    L = m.lib()
+    assert L.m3d_batch_destroy(None) == -1
+    assert L.m3d_batch_create(5, ct.byref(ct.c_void_p())) == -1

@@ -1,0 +1,87 @@
+"""runAnalysis "ep" / "ed" (basic scheme with mixed control, J2 and damage)
+on the GPU against the FP64 oracle, the golden fixture and the uniaxial J2
+closed form.  Bar: prop_homogenized and fields within 1e-9 relative, same
+iteration counts."""
+import os
+
+import numpy as np
+import pytest
+
+import mat3d_amd as m
+import oracle
+from mat3d_amd import gen
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+MAT_ED = np.array([[400e9, 0.20, np.inf, 0, 1, 1e-3, 2e3],
+                   [70e9, 0.30, np.inf, 0, 1, 1e-3, 8e2]])
+
+
+def run_both(shape, behav, mat, ids, E, X0, nInc, tol=1e-6):
+    o = oracle.Elastic(*shape, behav)
+    o.material(mat, ids)
+    o.set_load(E, X0)
+    it_o = o.run(tol, tol, nInc)
+    P_o = o.prop_homogenized()
+    eps_o, sig_o = o.fields()
+    s = m.FFT_Analysis3D(*shape, behav, precision="fp64")
+    s.material(mat, ids)
+    s.setLoad(E, X0)
+    info = s.runAnalysis(tol, tol, nInc)
+    eps, sig = s.fields()
+    return (P_o, eps_o, sig_o, it_o), (s.prop_homogenized, eps, sig, info)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (12, 16, 10), (9, 9, 9)])
+def test_j2_matches_oracle(shape):
+    n = max(shape)
+    ids = gen.spheres(n, 3, 0.25, seed=4)[: shape[0], : shape[1], : shape[2]].copy()
+    X0 = np.eye(6)[0]
+    (P_o, e_o, s_o, it_o), (P, e, s, info) = run_both(shape, "ep", gen.MAT_C4, ids, 0.02 * X0, X0, 8)
+    print("j2", shape, "iters oracle", it_o, "gpu", info["iters_fp64"], "last sigma", P[-1, 1])
+    assert info["iters_fp64"] == it_o
+    assert P.shape == P_o.shape
+    assert rel(P, P_o) < 1e-9
+    assert rel(s, s_o) < 1e-9 and rel(e, e_o) < 1e-9
+
+
+def test_damage_matches_oracle():
+    n = 16
+    ids = gen.spheres(n, 3, 0.25, seed=6)
+    X0 = np.eye(6)[1]
+    (P_o, e_o, s_o, it_o), (P, e, s, info) = run_both((n, n, n), "ed", MAT_ED, ids, 0.01 * X0, X0, 6)
+    print("damage iters", it_o, info["iters_fp64"], "d mean", P[:, 7])
+    assert info["iters_fp64"] == it_o
+    assert P.shape == P_o.shape
+    assert rel(P, P_o) < 1e-9
+    assert rel(s, s_o) < 1e-9
+
+
+def test_j2_golden_fixture():
+    f = np.load(os.path.join(G, "j2_8.npz"), allow_pickle=False)
+    s = m.FFT_Analysis3D(8, 8, 8, "ep", precision="fp64")
+    s.material(f["mat"], f["ids"])
+    s.setLoad(f["E"], f["X0"])
+    s.runAnalysis(1e-6, 1e-6, int(f["nInc"]))
+    assert rel(s.prop_homogenized, f["prop"]) < 1e-9
+
+
+def test_j2_uniaxial_closed_form_gpu():
+    n = 32
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    E, nu, sy, H = 70e9, 0.3, 100e6, 1e9
+    mat = np.array([[E, nu, sy, H, 1, 0, 0], [E, nu, sy, H, 1, 0, 0]])
+    s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
+    s.material(mat, ids)
+    X0 = np.eye(6)[0]
+    s.setLoad(0.004 * X0, X0)
+    s.runAnalysis(1e-10, 1e-10, 20)
+    for eps, s0 in s.prop_homogenized[:, :2]:
+        ref = E * eps if s0 <= sy else (eps + sy / H) / (1 / E + 1 / H)
+        assert abs(s0 - ref) < 1e-6 * max(abs(ref), sy)
