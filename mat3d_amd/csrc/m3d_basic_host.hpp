@@ -23,6 +23,7 @@ struct BasicRunner {
     int nblk = 256;
     std::vector<double> prop; // rows x 9, column-major like prop_homogenized
     int rows = 0;
+    int exit_inc = -1; // increment of the "ed" softening exit, -1 if none
     m3d_run_info info{};
 
     BasicRunner(Engine<Elastic>& eng, int b) : e(eng), beh(b)
@@ -170,6 +171,7 @@ struct BasicRunner {
             if (X0[i] != 0) { t = i; break; }
         prop.assign((size_t)9 * nInc, 0.0);
         rows = nInc;
+        exit_inc = -1;
         int nx = 0, total_it = 0;
         std::vector<double> part((size_t)14 * nblk);
         const int ck = 4;
@@ -239,8 +241,10 @@ struct BasicRunner {
             if (beh == BEH_ED && inc != 0)
                 for (int j = 0; j < 6; ++j)
                     if (X0[j] != 0 && prop[inc + (size_t)(j + 1) * R] < prop[inc - 1 + (size_t)(j + 1) * R]) nx += 1;
+            // softening exit: nInc = _i (:849); prop_homogenized keeps its
+            // nInc rows, the ones past the exit increment stay zero
             if (nx == 3) {
-                rows = inc;
+                exit_inc = inc;
                 break;
             }
             // update_old_Variable (:693-719); E += dE does not feed back (updateE recomputes E)

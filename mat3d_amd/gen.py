@@ -41,25 +41,60 @@ def _stamp_ball(ids, c, r, val):
     ids[ix, iy, iz] = val
 
 
+class _Centres:
+    """Centres and radii of placed balls with a periodic cell list, so that an
+    overlap query looks at the 27 neighbouring cells only (same decisions as
+    comparing against every centre)."""
+
+    def __init__(self, n, reach):
+        self.n = n
+        self.nc = max(1, n // max(1, int(np.ceil(reach))))
+        self.cells = {}
+        self.count = 0
+
+    def _cell(self, c):
+        return tuple(int(v) * self.nc // self.n for v in c)
+
+    def overlaps(self, c, r):
+        if self.count == 0:
+            return False
+        if self.nc < 3:
+            keys = list(self.cells.keys())
+        else:
+            k = self._cell(c)
+            keys = {((k[0] + i) % self.nc, (k[1] + j) % self.nc, (k[2] + l) % self.nc)
+                    for i in (-1, 0, 1) for j in (-1, 0, 1) for l in (-1, 0, 1)}
+        for key in keys:
+            lst = self.cells.get(key)
+            if not lst:
+                continue
+            arr = np.array(lst, dtype=np.int64)
+            d = _wrap(arr[:, :3] - c, self.n)
+            if np.any((d * d).sum(1) <= (arr[:, 3] + r) ** 2):
+                return True
+        return False
+
+    def add(self, c, r):
+        self.cells.setdefault(self._cell(c), []).append((int(c[0]), int(c[1]), int(c[2]), int(r)))
+        self.count += 1
+
+
 def spheres(n, radius, frac, seed, incl=0, matrix=1, max_tries=2_000_000):
     """Random sequential addition of non-overlapping balls |d|^2 <= r^2 (integer
     centres, periodic) until the inclusion fraction reaches ``frac``."""
     rng = np.random.default_rng(seed)
     ids = np.full((n, n, n), matrix, dtype=np.int32)
-    centres = np.zeros((0, 3), dtype=np.int64)
+    placed = _Centres(n, 2 * radius + 1)
     target = frac * n**3
     count = 0
-    vol1 = int(((np.arange(-radius, radius + 1)[:, None, None] ** 2 + np.arange(-radius, radius + 1)[None, :, None] ** 2
-                 + np.arange(-radius, radius + 1)[None, None, :] ** 2) <= radius * radius).sum())
+    vol1 = _ball_volume(radius)
     tries = 0
     while count < target and tries < max_tries:
         tries += 1
         c = rng.integers(0, n, size=3)
-        if len(centres):
-            d = _wrap(centres - c, n)
-            if np.any((d * d).sum(1) <= (2 * radius) ** 2):
-                continue
-        centres = np.vstack([centres, c])
+        if placed.overlaps(c, radius):
+            continue
+        placed.add(c, radius)
         _stamp_ball(ids, c, radius, incl)
         count += vol1
     return ids
@@ -70,8 +105,7 @@ def multi_spheres(n, groups, seed, matrix=1, max_tries=5_000_000):
     (phase, fraction, rmin, rmax).  Radii are drawn uniformly (integer)."""
     rng = np.random.default_rng(seed)
     ids = np.full((n, n, n), matrix, dtype=np.int32)
-    cs = np.zeros((0, 3), dtype=np.int64)
-    rs = np.zeros((0,), dtype=np.int64)
+    placed = _Centres(n, 2 * max(g[3] for g in groups) + 1)
     for phase, frac, rmin, rmax in groups:
         target = frac * n**3
         got = 0
@@ -80,12 +114,9 @@ def multi_spheres(n, groups, seed, matrix=1, max_tries=5_000_000):
             tries += 1
             r = int(rng.integers(rmin, rmax + 1))
             c = rng.integers(0, n, size=3)
-            if len(cs):
-                d = _wrap(cs - c, n)
-                if np.any((d * d).sum(1) <= (rs + r) ** 2):
-                    continue
-            cs = np.vstack([cs, c])
-            rs = np.append(rs, r)
+            if placed.overlaps(c, r):
+                continue
+            placed.add(c, r)
             _stamp_ball(ids, c, r, phase)
             got += _ball_volume(r)
     return ids
