@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 GPU comparison")
     ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
+                         "reference for HBM byte counters, then exit")
     ap.add_argument("--workload", default="fibres256",
                     choices=["fibres256", "voronoi_batch", "thermal512", "j2_192"],
                     help="fibres256 is the headline line; the others measure the batch, thermal and "
@@ -92,6 +95,8 @@ def main():
               "sharding the load cases over several GPUs is not launched by this build.", flush=True)
         sys.exit(2)
 
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.workload != "fibres256":
         return OTHER[args.workload](args)
 
@@ -192,6 +197,20 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(gib):
+    """A fill (writes B bytes) and three copies (read B, write B each) of
+    B = gib GiB, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = int(gib * 2**30) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: fill + 3 copies of {4 * n} bytes", flush=True)
 
 
 def _line(args, metric, value, unit, ms, dtype, workload, extra):

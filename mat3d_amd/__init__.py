@@ -22,7 +22,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmat3d.so")
+# M3D_LIB selects another build of the library (e.g. one with other tuning macros)
+LIB_PATH = os.environ.get("M3D_LIB") or os.path.join(_HERE, "libmat3d.so")
 
 M3D_OK = 0
 ERRORS = {-1: "M3D_ERR_ARG", -2: "M3D_ERR_UNSUPPORTED", -3: "M3D_ERR_OOM", -4: "M3D_ERR_HIP",

@@ -30,7 +30,7 @@ struct m3d_batch {
     std::vector<BatchProblem> probs;
     bool ran = false;
     double polish_factor = 10.0, fp32_floor = 1e-6;
-    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0;
+    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1;
     double ms = 0, voxel_its = 0;
 };
 
@@ -113,6 +113,7 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
         else if (k == "rollback" && value >= 0 && value <= 64) h->eng->rollback = (int)value;
         else if (k == "cap" && value >= 1) h->eng->cap = (int)value;
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
+        else if (k == "refine" && (value == 0 || value == 1)) h->eng->refine = (int)value;
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
     });
@@ -285,6 +286,7 @@ int m3d_batch_set_option(m3d_batch* b, const char* key, double value)
         else if (k == "cap" && value >= 1) b->cap = (int)value;
         else if (k == "chunk" && value >= 0) b->chunk = (int)value;
         else if (k == "max_solves" && value >= 0) b->max_solves = (int)value;
+        else if (k == "refine" && (value == 0 || value == 1)) b->refine = (int)value;
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
     });
@@ -337,6 +339,7 @@ int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
             e.rollback = b->rollback;
             e.cap = b->cap;
             e.chunk = b->chunk;
+            e.refine = b->refine;
             // RVEs per device batch: 70% of free memory, or the max_solves option
             size_t fr = 0, tot = 0;
             HCHK(hipMemGetInfo(&fr, &tot));

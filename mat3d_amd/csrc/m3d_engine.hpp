@@ -231,6 +231,7 @@ struct Engine {
     long long N = 0, H = 0;
     // device buffers
     float* U32 = nullptr;  // ring of rollback+1 FP32 iterates x solves
+    float* R32 = nullptr;  // anchored polish: r = T(A) - A per solve (FP32)
     double* U64 = nullptr; // two FP64 iterates x solves
     int ring32 = 0;        // ring size the FP32 buffer was allocated with
     unsigned char* Sbuf = nullptr;
@@ -256,6 +257,7 @@ struct Engine {
     std::vector<int> final_prec; // per solve: 0 fp32, 1 fp64
     std::vector<m3d_run_info> info;
     double polish_factor = 10.0;
+    int refine = 1; // anchored FP32 polish (1) or plain FP64 polish iterations (0)
     int rollback = 5;
     double rollback_margin = 2.0;
     double fp32_floor = 1e-6;
@@ -289,8 +291,10 @@ struct Engine {
     void free_solves()
     {
         if (U32) hipFree(U32);
+        if (R32) hipFree(R32);
         if (U64) hipFree(U64);
         U32 = nullptr;
+        R32 = nullptr;
         U64 = nullptr;
         if (Sbuf) hipFree(Sbuf);
         if (Kbuf) hipFree(Kbuf);
@@ -343,7 +347,7 @@ struct Engine {
     {
         const size_t nc = (size_t)N * C, hc = (size_t)H * C;
         size_t b = sizeof(double) * nc * 2 + 2 * sizeof(cx<double>) * hc;
-        if (precision == M3D_FP32_POLISH) b += sizeof(float) * nc * ring_needed();
+        if (precision == M3D_FP32_POLISH) b += sizeof(float) * nc * (ring_needed() + 1);
         return b;
     }
 
@@ -356,6 +360,7 @@ struct Engine {
         if (precision == M3D_FP32_POLISH) {
             ring32 = ring_needed();
             alloc(&U32, sizeof(float) * nc * n * ring32);
+            alloc(&R32, sizeof(float) * nc * n);
         }
         alloc(&U64, sizeof(double) * nc * n * 2);
         alloc(&Sbuf, sizeof(cx<double>) * hc * n);
@@ -404,6 +409,10 @@ struct Engine {
         p.psum = psum;
         p.pchk = pchk;
         p.ctl = ctl;
+        p.R = R32;
+        p.eU = U32;
+        p.eB = (long long)nc * cap_solves;
+        p.ering = ring32 > 0 ? ring32 : 1;
         (void)g;
         return p;
     }
@@ -443,8 +452,10 @@ struct Engine {
         }
         const size_t zsm = (size_t)g.TY * C * g.lsz * sizeof(cx<T>);
         const dim3 grid(g.nsolve * g.nbz);
-        if (g.gen) hipLaunchKernelGGL((k_zpass<T, PH, MODE, true>), grid, dim3(ZTHREADS), zsm, st, g, p);
-        else hipLaunchKernelGGL((k_zpass<T, PH, MODE, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
+        // the generic kernel adds r of anchored solves at run time
+        constexpr int GM = MODE == MODE_IUFR ? MODE_IUF : MODE;
+        if (g.gen) hipLaunchKernelGGL((k_zpass<T, PH, GM, true>), grid, dim3(ZTHREADS), zsm, st, g, p);
+        else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
 
     // KIND 0: y pass, 1: x pass + Green, 2: x pass of the check FFT
@@ -552,19 +563,42 @@ struct Engine {
         return 8;
     }
 
-    // run the iteration in precision T until every solve of the batch is done
-    template <typename T>
-    void run_phase(Geo g, std::vector<Ctl>& hctl)
+    // anchored polish: FP32 passes on the corrections e (r added in the
+    // update), equilibrium checks in FP64 on anchor + e
+    void launch_start_corr(const Geo& gf, const Ptr<float>& pf)
     {
-        g.nsolve = nsolve;
-        Ptr<T> p = ptrs<T>(g);
-        launch_start<T>(g, p);
-        HCHK(hipGetLastError());
-        const int ck = chunk_for(g);
+        zpass<float, MODE_FWD>(gf, pf);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(gf.nsolve), dim3(FTHREADS), 0, st, gf, pf.ctl, pf.psum);
+        ypass<float, -1, BUF_MAIN>(gf, pf);
+        xgreen<float>(gf, pf);
+        ypass<float, 1, BUF_MAIN>(gf, pf);
+    }
+
+    void launch_iter_corr(const Geo& gf, const Ptr<float>& pf, const Geo& gd, const Ptr<double>& pd)
+    {
+        zpass<float, MODE_IUFR>(gf, pf);
+        hipLaunchKernelGGL((k_finA<C>), dim3(gf.nsolve), dim3(FTHREADS), 0, st, gf, pf.ctl, pf.psum);
+        zpass<double, MODE_CHK_CUR>(gd, pd);
+        ypass<double, -1, BUF_CHK_CUR>(gd, pd);
+        xcheck<double, BUF_CHK_CUR>(gd, pd);
+        zpass<double, MODE_CHK_PREV>(gd, pd);
+        ypass<double, -1, BUF_CHK_PREV>(gd, pd);
+        xcheck<double, BUF_CHK_PREV>(gd, pd);
+        hipLaunchKernelGGL((k_finB<C>), dim3(gd.nsolve), dim3(FTHREADS), 0, st, gd, pd.ctl, pd.pchk);
+        ypass<float, -1, BUF_MAIN>(gf, pf);
+        xgreen<float>(gf, pf);
+        ypass<float, 1, BUF_MAIN>(gf, pf);
+    }
+
+    // capture `ck` iterations (iter()) in a graph and replay it until every
+    // solve of the batch is done; the host polls the done flags in between
+    template <typename Iter>
+    void run_graph(int ck, Iter&& iter, std::vector<Ctl>& hctl)
+    {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < ck; ++i) launch_iter<T>(g, p);
+        for (int i = 0; i < ck; ++i) iter();
         hipError_t ce = hipStreamEndCapture(st, &graph);
         if (ce != hipSuccess) throw HipError{ce, "hipStreamEndCapture"};
         ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -588,6 +622,29 @@ struct Engine {
         }
         hipGraphExecDestroy(exec);
         hipGraphDestroy(graph);
+    }
+
+    // run the iteration in precision T until every solve of the batch is done
+    template <typename T>
+    void run_phase(Geo g, std::vector<Ctl>& hctl)
+    {
+        g.nsolve = nsolve;
+        Ptr<T> p = ptrs<T>(g);
+        launch_start<T>(g, p);
+        HCHK(hipGetLastError());
+        run_graph(chunk_for(g), [&]() { launch_iter<T>(g, p); }, hctl);
+    }
+
+    // anchored polish of the solves flagged corr (see k_anchor)
+    void run_corr(Geo gf, Geo gd, std::vector<Ctl>& hctl)
+    {
+        gf.nsolve = gd.nsolve = nsolve;
+        Ptr<float> pf = ptrs<float>(gf);
+        Ptr<double> pd = ptrs<double>(gd);
+        hipLaunchKernelGGL((k_anchor<C>), dim3(1024), dim3(256), 0, st, gd, pd, pf, ctl);
+        launch_start_corr(gf, pf);
+        HCHK(hipGetLastError());
+        run_graph(chunk_for(gf), [&]() { launch_iter_corr(gf, pf, gd, pd); }, hctl);
     }
 
     // loads: nsolve x 6 (only the first C entries used); tol per phase
@@ -654,7 +711,32 @@ struct Engine {
             hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, ptrs<float>(gf), ptrs<double>(gd), ctl);
             bool any = false;
             for (int s = 0; s < nsolve; ++s) any = any || (hctl[s].done == 0);
-            if (any) run_phase<double>(gd, hctl);
+            if (any && refine) {
+                // one FP64 iteration from the switch iterate A gives r = T(A) - A;
+                // the rest of the polish iterates the FP32 correction e <- r + M e
+                // (the load term drops out of M) with every stop-rule quantity
+                // taken in FP64 from A + e
+                Ptr<double> pd = ptrs<double>(gd);
+                launch_start<double>(gd, pd);
+                launch_iter<double>(gd, pd);
+                HCHK(hipGetLastError());
+                HCHK(hipMemcpyAsync(hctl.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost, st));
+                HCHK(hipStreamSynchronize(st));
+                bool left = false;
+                for (int s = 0; s < nsolve; ++s) {
+                    if (hctl[s].done) continue;
+                    hctl[s].corr = 1;
+                    hctl[s].anc_it = hctl[s].it - 1;
+                    for (int k = 0; k < 6; ++k) hctl[s].E[k] = 0.0;
+                    left = true;
+                }
+                if (left) {
+                    HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
+                    run_corr(gf, gd, hctl);
+                }
+            } else if (any) {
+                run_phase<double>(gd, hctl);
+            }
             for (int s = 0; s < nsolve; ++s) info[s].iters_fp64 = hctl[s].it - jr[s];
         } else {
             Ptr<double> pd = ptrs<double>(gd);
@@ -747,6 +829,7 @@ struct Engine {
             c.done = 0;
             c.need_cur = 1;
             c.need_prev = 1;
+            c.corr = 0;
         }
         HCHK(hipMemcpy(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice));
         for (int k = 0; k < 6; ++k) {

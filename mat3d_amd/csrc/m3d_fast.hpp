@@ -76,6 +76,25 @@ constexpr int xytile(int L, int C)
 constexpr int ZNT = 256;
 constexpr int XYNT = 256;
 
+// minimum waves per SIMD the register allocator must leave room for (0: no
+// constraint); these passes are HBM-bound, so occupancy buys latency hiding
+#ifndef M3D_KZ_WAVES
+#define M3D_KZ_WAVES 4
+#endif
+#ifndef M3D_KXY_WAVES
+#define M3D_KXY_WAVES 0
+#endif
+#if M3D_KZ_WAVES > 0
+#define M3D_KZ_ATTR __attribute__((amdgpu_waves_per_eu(M3D_KZ_WAVES)))
+#else
+#define M3D_KZ_ATTR
+#endif
+#if M3D_KXY_WAVES > 0
+#define M3D_KXY_ATTR __attribute__((amdgpu_waves_per_eu(M3D_KXY_WAVES)))
+#else
+#define M3D_KXY_ATTR
+#endif
+
 // XCD-aware block order: the dispatcher hands block b to XCD b % 8; give
 // each XCD a contiguous run of logical tiles so tiles sharing cache lines
 // run under one L2.
@@ -305,7 +324,7 @@ __device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T
 // Block: solve s, plane x, lines y0..y0+TY-1.  LDS: (LZ+1)*W complex lines
 // (W = TY*C, line l = yl*C + c) + the Nz twiddle table.
 template <typename T, typename PH, int MODE, int LZ, int TY>
-__global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
+__global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
 {
     constexpr int C = PH::C;
     constexpr int W = TY * C;
@@ -323,6 +342,8 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
     const int t = b - s * per;
     const int x = t / (g.Ny / TY);
     const int y0 = (t - x * (g.Ny / TY)) * TY;
+    constexpr bool IUF = MODE == MODE_IUF || MODE == MODE_IUFR;
+    constexpr bool CHK = MODE == MODE_CHK_CUR || MODE == MODE_CHK_PREV;
     const Ctl& ctl = p.ctl[s];
     if (ctl.done) return;
     if (MODE == MODE_CHK_CUR && !ctl.need_cur) return;
@@ -338,8 +359,17 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
     const T* Uin;
     T* Uout = nullptr;
     if (MODE == MODE_FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
-    else if (MODE == MODE_IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
+    else if (IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
     else Uin = u_buf(p, it - 1, s);
+    // anchored polish: the FP64 check reads anchor + FP32 correction
+    const float* Ein = nullptr;
+    if constexpr (sizeof(T) == 8 && CHK) {
+        if (ctl.corr) {
+            Uin = u_buf(p, ctl.anc_it, s);
+            Ein = e_buf(p, MODE == MODE_CHK_CUR ? it : it - 1, s) + vox0 * C;
+        }
+    }
+    const float* Rin = MODE == MODE_IUFR ? p.R + (long long)s * p.sU + vox0 * C : nullptr;
     Uin += vox0 * C;
     if (Uout) Uout += vox0 * C;
 
@@ -350,8 +380,9 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
     using ZC = ZChunk<T, TY, NZH, C, NT>;
     typename ZC::V chunk[ZC::PER];
-    if (MODE == MODE_IUF) zchunk_issue<ZC>(chunk, p.S + (long long)s * p.sS + spec0);
+    if (IUF) zchunk_issue<ZC>(chunk, p.S + (long long)s * p.sS + spec0);
     T u[QI][2 * C];
+    T rr[MODE == MODE_IUFR ? QI : 1][2 * C];
     uint16_t pp[QI];
 #pragma unroll
     for (int i = 0; i < QI; ++i) {
@@ -360,11 +391,20 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
             const int yl = q / LZ, m = q - yl * LZ;
             const int v0 = yl * NZ + 2 * m;
             vload<T, 2 * C>(Uin + (long long)v0 * C, u[i]);
+            if constexpr (MODE == MODE_IUFR) vload<T, 2 * C>(Rin + (long long)v0 * C, rr[i]);
             pp[i] = *reinterpret_cast<const uint16_t*>(ph + v0);
+            if constexpr (sizeof(T) == 8 && CHK) {
+                if (Ein) {
+                    float e2[2 * C];
+                    vload<float, 2 * C>(Ein + (long long)v0 * C, e2);
+#pragma unroll
+                    for (int c = 0; c < 2 * C; ++c) u[i][c] += (T)e2[c];
+                }
+            }
         }
     }
 
-    if (MODE == MODE_IUF) {
+    if (IUF) {
         zchunk_commit<ZC, NZH, C>(chunk, sl);
         __syncthreads();
         // c2r pre-processing; kz=0 and kz=Nz/2 keep their real parts only
@@ -407,7 +447,7 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
             const T* k0 = coef + (int)(pp[i] & 0xff) * NCOEF;
             const T* k1 = coef + (int)(pp[i] >> 8) * NCOEF;
             cx<T>* slot = sl + m * W + yl * C;
-            if (MODE == MODE_IUF) {
+            if (IUF) {
                 T d0[C], d1[C], a0[C], a1[C];
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
@@ -422,12 +462,16 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
                 for (int c = 0; c < C; ++c) {
                     u0[c] += a0[c];
                     u1[c] += a1[c];
+                    if constexpr (MODE == MODE_IUFR) {
+                        u0[c] += rr[i][c];
+                        u1[c] += rr[i][C + c];
+                    }
                     un[c] = u0[c];
                     un[C + c] = u1[c];
                 }
                 vstore<T, 2 * C>(Uout + (long long)v0 * C, un);
             }
-            if (MODE == MODE_FWD || MODE == MODE_IUF) {
+            if (MODE == MODE_FWD || IUF) {
                 PH::dual(k0, u0, o0);
                 PH::dual(k1, u1, o1);
 #pragma unroll
@@ -445,7 +489,7 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
             for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
         }
     }
-    if (MODE == MODE_FWD || MODE == MODE_IUF) {
+    if (MODE == MODE_FWD || IUF) {
         const double bs = block_sum(acc, red);
         if (tid == 0) p.psum[(long long)s * g.nbz + t] = bs;
     }
@@ -472,7 +516,7 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
     }
     __syncthreads();
     {
-        cx<T>* dst = ((MODE == MODE_FWD || MODE == MODE_IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
+        cx<T>* dst = ((MODE == MODE_FWD || IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
         lds_to_zchunk<T, TY, NZH, C, NT>(sl, dst);
     }
 }
@@ -480,7 +524,7 @@ __global__ void __launch_bounds__(ZNT) kz(Geo g, Ptr<T> p)
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
 template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK>
-__global__ void __launch_bounds__(XYNT) kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+__global__ void __launch_bounds__(XYNT) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;

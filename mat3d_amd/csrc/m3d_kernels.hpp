@@ -36,7 +36,9 @@ constexpr int ZTHREADS = 256;
 constexpr int XYTHREADS = 512;
 constexpr int EPT = 16;  // complex values per thread per pass
 
-enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3 };
+// z-pass modes: forward only, update (inverse, update, forward), check FFT of
+// the current / previous iterate, update of an anchored FP32 correction (adds r)
+enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3, MODE_IUFR = 4 };
 enum { BUF_MAIN = 0, BUF_CHK_CUR = 2, BUF_CHK_PREV = 3, BUF_K = 4 };
 
 // per-solve control block, read and written on the device only
@@ -50,7 +52,13 @@ struct Ctl {
     int first;     // basic scheme: 1 while the stage after the increment start runs
     int err;       // basic scheme: constitutive update failed (Newton NaN / no convergence)
     int sb;        // elastic: 1 strain base, 0 stress base; thermal: 1 gradient base
+    // anchored polish: the iterate is A + e with the FP64 anchor A (iterate
+    // anc_it of the FP64 ring) and an FP32 correction e (FP32 ring); the
+    // FP32 passes iterate e <- r + M e (r = T(A) - A, load term dropped)
+    int corr;
+    int anc_it;
     int pad;
+    double sum_anchor; // sum of the dual field of the anchor (FP64)
     double sum_prev;  // sum of the dual field at iterate `it`
     double eq_val;    // eqVar at iterate eq_it
     double dsum, deq, eqvar;
@@ -101,6 +109,13 @@ struct Ptr {
     double* psum;           // z-pass partial sums, [nsolve][nbz]
     double* pchk;           // check partials, [nsolve][2][nbx + 1]
     Ctl* ctl;
+    // anchored polish (solves with ctl.corr): FP32 passes add R to the
+    // update; FP64 passes read the iterate as U[anc_it] + e, e from the FP32
+    // ring eU (stride eB between ring buffers, `ering` buffers)
+    const float* R;
+    const float* eU;
+    long long eB;
+    int ering;
 };
 
 // ----------------------------------------------------------------- physics
@@ -254,6 +269,15 @@ __device__ __forceinline__ T* u_buf(const Ptr<T>& p, int it, int s)
     return p.U + (long long)b * p.sB + (long long)s * p.sU;
 }
 
+// FP32 correction e of iterate `it` of an anchored solve
+template <typename T>
+__device__ __forceinline__ const float* e_buf(const Ptr<T>& p, int it, int s)
+{
+    int b = it % p.ering;
+    if (b < 0) b += p.ering;
+    return p.eU + (long long)b * p.eB + (long long)s * p.sU;
+}
+
 // --------------------------------------------------------------- z pass
 // One block: solve s, plane x, lines y0..y0+TY-1, all components.
 // LDS: TY*C lines of lsz complex each; line l = yl*C + c.
@@ -291,6 +315,17 @@ __global__ void __launch_bounds__(ZTHREADS) k_zpass(Geo g, Ptr<T> p)
     else if (MODE == MODE_IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
     else if (MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
     else Uin = u_buf(p, it - 1, s);
+    // anchored polish: FP32 updates add r, FP64 checks read anchor + e
+    const bool corr = ctl.corr != 0;
+    const float* Rin = nullptr;
+    const float* Ein = nullptr;
+    if (corr) {
+        if (sizeof(T) == 4 && MODE == MODE_IUF) Rin = p.R + (long long)s * p.sU + vox0 * C;
+        if (sizeof(T) == 8 && (MODE == MODE_CHK_CUR || MODE == MODE_CHK_PREV)) {
+            Uin = u_buf(p, ctl.anc_it, s);
+            Ein = e_buf(p, MODE == MODE_CHK_CUR ? it : it - 1, s) + vox0 * C;
+        }
+    }
     Uin += vox0 * C;
     if (Uout) Uout += vox0 * C;
 
@@ -375,16 +410,23 @@ __global__ void __launch_bounds__(ZTHREADS) k_zpass(Geo g, Ptr<T> p)
         T u[C], o[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) u[c] = Uin[(long long)v * C + c];
+        if (Ein) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] += (T)Ein[(long long)v * C + c];
+        }
         if (MODE == MODE_IUF) {
             T d[C], a[C];
 #pragma unroll
             for (int c = 0; c < C; ++c) d[c] = sr[(yl * C + c) * 2 * lsz + z] - u[c];
             PH::accel(kp, d, a);
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                u[c] = u[c] + a[c];
-                Uout[(long long)v * C + c] = u[c];
+            for (int c = 0; c < C; ++c) u[c] = u[c] + a[c];
+            if (Rin) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) u[c] += (T)Rin[(long long)v * C + c];
             }
+#pragma unroll
+            for (int c = 0; c < C; ++c) Uout[(long long)v * C + c] = u[c];
         }
         if (MODE == MODE_FWD || MODE == MODE_IUF) {
             PH::dual(kp, u, o);
@@ -684,7 +726,12 @@ __global__ void __launch_bounds__(FTHREADS) k_fin0(Geo g, Ctl* ctl, const double
     if (done) return;
     const double S = ordered_sum(psum + (long long)s * g.nbz, g.nbz, red);
     if (threadIdx.x == 0) {
-        c.sum_prev = S;
+        // anchored solves sum e; the anchor's sum completes the iterate's
+        if (c.corr) c.sum_prev = S + c.sum_anchor;
+        else {
+            c.sum_prev = S;
+            c.sum_anchor = S;
+        }
         c.eq_it = -1;
         c.need_cur = 0;
         c.need_prev = 0;
@@ -702,8 +749,9 @@ __global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double
     const int done = c.done;
     __syncthreads();
     if (done) return;
-    const double S = ordered_sum(psum + (long long)s * g.nbz, g.nbz, red);
+    const double S0 = ordered_sum(psum + (long long)s * g.nbz, g.nbz, red);
     if (threadIdx.x == 0) {
+        const double S = c.corr ? S0 + c.sum_anchor : S0;
         c.it += 1;
         c.dsum = fabs((S - c.sum_prev) / c.norm);
         c.sum_prev = S;
@@ -771,6 +819,29 @@ __global__ void k_promote(Geo g, Ptr<float> p32, Ptr<double> p64, const Ctl* ctl
     }
 }
 
+// start of the anchored polish, after one FP64 iteration from the anchor
+// A = iterate it-1: r = T(A) - A = u[it] - u[it-1] (FP64 difference, kept
+// in FP32), and the correction of iterate `it` is e = r.  Skips solves that
+// are done or not anchored.
+template <int C>
+__global__ void k_anchor(Geo g, Ptr<double> p64, Ptr<float> p32, const Ctl* ctl)
+{
+    const long long n = g.N * C;
+    for (int s = 0; s < g.nsolve; ++s) {
+        if (ctl[s].done || !ctl[s].corr) continue;
+        const int it = ctl[s].it;
+        const double* a = u_buf(p64, it - 1, s);
+        const double* b = u_buf(p64, it, s);
+        float* e = u_buf(p32, it, s);
+        float* r = const_cast<float*>(p32.R) + (long long)s * p32.sU;
+        for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+            const float d = (float)(b[i] - a[i]);
+            e[i] = d;
+            r[i] = d;
+        }
+    }
+}
+
 // means of u and of the dual field v = M(x) u at the final iterate; output
 // per solve [2*C] partial sums per block
 template <typename T, typename PH>
@@ -779,7 +850,10 @@ __global__ void __launch_bounds__(256) k_means(Geo g, Ptr<T> p, double* out, int
     __shared__ double red[256 / 64];
     constexpr int C = PH::C;
     const int s = blockIdx.y;
-    const T* U = u_buf(p, p.ctl[s].it, s);
+    const Ctl& ctl = p.ctl[s];
+    const bool corr = sizeof(T) == 8 && ctl.corr;
+    const T* U = u_buf(p, corr ? ctl.anc_it : ctl.it, s);
+    const float* E = corr ? e_buf(p, ctl.it, s) : nullptr;
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh;
     const T* coef = p.coef + (long long)rv * p.sCoef;
@@ -790,6 +864,10 @@ __global__ void __launch_bounds__(256) k_means(Geo g, Ptr<T> p, double* out, int
         T u[C], o[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) u[c] = U[v * C + c];
+        if (E) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] += (T)E[v * C + c];
+        }
         PH::dual(coef + (int)ph[v] * NCOEF, u, o);
 #pragma unroll
         for (int c = 0; c < C; ++c) { su[c] += (double)u[c]; sv[c] += (double)o[c]; }
@@ -807,7 +885,10 @@ template <typename T, typename PH>
 __global__ void k_fields(Geo g, Ptr<T> p, int s, double* uo, double* vo)
 {
     constexpr int C = PH::C;
-    const T* U = u_buf(p, p.ctl[s].it, s);
+    const Ctl& ctl = p.ctl[s];
+    const bool corr = sizeof(T) == 8 && ctl.corr;
+    const T* U = u_buf(p, corr ? ctl.anc_it : ctl.it, s);
+    const float* E = corr ? e_buf(p, ctl.it, s) : nullptr;
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh;
     const T* coef = p.coef + (long long)rv * p.sCoef;
@@ -815,6 +896,10 @@ __global__ void k_fields(Geo g, Ptr<T> p, int s, double* uo, double* vo)
         T u[C], o[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) u[c] = U[v * C + c];
+        if (E) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] += (T)E[v * C + c];
+        }
         PH::dual(coef + (int)ph[v] * NCOEF, u, o);
 #pragma unroll
         for (int c = 0; c < C; ++c) {

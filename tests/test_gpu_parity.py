@@ -145,6 +145,52 @@ def test_thermal_nphase_fp32_polish():
     assert err < 1e-6
 
 
+@pytest.mark.parametrize("case", ["elastic32", "elastic_generic", "thermal32"])
+def test_anchored_polish_matches_fp64_polish(case):
+    """FP32 + polish with the anchored FP32 correction iterations (refine=1,
+    the default) against plain FP64 polish iterations (refine=0) and the
+    oracle: same iteration counts, same C_eff / K_eff to FP32-switch level."""
+    if case == "thermal32":
+        n = 32
+        shape = (n, n, n)
+        ids = gen.multi_spheres(n, [(2, 0.15, 2, 4), (0, 0.10, 2, 4)], seed=5)
+        k = np.array([0.01, 1.0, 1000.0])
+        o = oracle.Thermal(*shape)
+        o.material_n(k, ids)
+        r = o.homogenization(1e-7, 1e-7)
+        ref, iters_o = r["K"], r["iters"]
+        out = {}
+        for refine in (1, 0):
+            s = m.FFT_analysis3d_thermal(*shape, precision="fp32_polish")
+            s.set_option("refine", refine)
+            s.material_n(k, ids)
+            s.homogenization(1e-7, 1e-7)
+            out[refine] = (s.KHomo.copy(), [i["iterations"] for i in s.info], s.fields(1)[1])
+            s.close()
+        q_o = None
+    else:
+        shape = (32, 32, 32) if case == "elastic32" else (12, 16, 10)
+        ids = gen.fibres(32, 3, 0.3, seed=2, length=16)[: shape[0], : shape[1], : shape[2]].copy()
+        mat = gen.MAT_C2
+        o, r = oracle_homog(shape, mat, ids)
+        ref, iters_o = r["C"], r["iters"]
+        out = {}
+        for refine in (1, 0):
+            s = m.FFT_Analysis3D(*shape, "e", precision="fp32_polish")
+            s.set_option("refine", refine)
+            s.material(mat, ids)
+            s.homogenization(1e-6, 1e-6)
+            out[refine] = (s.CHomo.copy(), [i["iterations"] for i in s.info], s.fields(3)[1])
+            s.close()
+    for refine in (1, 0):
+        C, its, _ = out[refine]
+        print(f"[{case}] refine={refine} rel err {rel(C, ref):.3e} iters {its} oracle {iters_o}")
+        assert rel(C, ref) < 1e-6
+    assert out[1][1] == out[0][1]
+    assert rel(out[1][0], out[0][0]) < 1e-6
+    assert field_rel(out[1][2], out[0][2]) < 1e-5
+
+
 # ---------------------------------------------------------------------------
 # compile-time specialised kernels: fixed iteration counts against the oracle
 def _fixed_iters_elastic(shape, ids, mat, w, k, precision):
