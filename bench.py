@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 GPU comparison")
     ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="solver option for the FP32+polish handle (m3d_elastic_set_option), repeatable")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
                          "reference for HBM byte counters, then exit")
@@ -112,6 +114,9 @@ def main():
     print(f"[bench] generated {n}^3 fibre RVE in {t_gen:.1f}s, fibre fraction {(ids == 0).mean():.4f}", flush=True)
 
     s = m.FFT_Analysis3D(n, n, n, "e", precision="fp32_polish")
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        s.set_option(k, float(v))
     s.material(mat, ids)
     for i in range(args.warmup):
         s.homogenization(args.tol, args.tol)

@@ -24,10 +24,22 @@ struct BatchProblem {
     m3d_run_info info[6];
 };
 
+// problems of one grid shape: one engine, kept across calls; `loaded`: the
+// group fits one device batch and its materials are on the device
+struct BatchGroup {
+    std::array<int, 3> shape;
+    std::vector<int> members;
+    Engine<Elastic>* eng = nullptr;
+    long long per = 0; // RVEs per device batch
+    bool loaded = false;
+};
+
 struct m3d_batch {
     int precision = M3D_FP32_POLISH;
     std::string err;
     std::vector<BatchProblem> probs;
+    std::vector<BatchGroup> groups; // rebuilt after m3d_batch_add
+    bool grouped = false;
     bool ran = false;
     double polish_factor = 10.0, fp32_floor = 1e-6;
     int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1;
@@ -267,9 +279,19 @@ int m3d_batch_create(int precision, m3d_batch** out)
     return M3D_OK;
 }
 
+namespace {
+void batch_free_groups(m3d_batch* b)
+{
+    for (BatchGroup& G : b->groups) delete G.eng;
+    b->groups.clear();
+    b->grouped = false;
+}
+} // namespace
+
 int m3d_batch_destroy(m3d_batch* b)
 {
     if (!b) return M3D_ERR_ARG;
+    batch_free_groups(b);
     delete b;
     return M3D_OK;
 }
@@ -285,7 +307,10 @@ int m3d_batch_set_option(m3d_batch* b, const char* key, double value)
         else if (k == "rollback" && value >= 0 && value <= 64) b->rollback = (int)value;
         else if (k == "cap" && value >= 1) b->cap = (int)value;
         else if (k == "chunk" && value >= 0) b->chunk = (int)value;
-        else if (k == "max_solves" && value >= 0) b->max_solves = (int)value;
+        else if (k == "max_solves" && value >= 0) {
+            b->max_solves = (int)value;
+            batch_free_groups(b); // device batch sizes change
+        }
         else if (k == "refine" && (value == 0 || value == 1)) b->refine = (int)value;
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
@@ -309,6 +334,7 @@ int m3d_batch_add(m3d_batch* b, int nx, int ny, int nz, const double* mat, const
         elastic_props("e", mat, rve, nph, N, q.P, q.ids.data());
         b->probs.push_back(std::move(q));
         b->ran = false;
+        batch_free_groups(b);
         if (index) *index = (int)b->probs.size() - 1;
         return M3D_OK;
     });
@@ -324,49 +350,65 @@ int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
         b->ran = false;
         b->ms = 0;
         b->voxel_its = 0;
-        std::vector<std::array<int, 3>> shapes;
-        for (const BatchProblem& q : b->probs) {
-            std::array<int, 3> sh{q.nx, q.ny, q.nz};
-            if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
+        if (!b->grouped) {
+            batch_free_groups(b);
+            for (int i = 0; i < (int)b->probs.size(); ++i) {
+                const std::array<int, 3> sh{b->probs[i].nx, b->probs[i].ny, b->probs[i].nz};
+                auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BatchGroup& G) { return G.shape == sh; });
+                if (it == b->groups.end()) {
+                    b->groups.push_back(BatchGroup{});
+                    b->groups.back().shape = sh;
+                    it = b->groups.end() - 1;
+                }
+                it->members.push_back(i);
+            }
+            b->grouped = true;
         }
-        for (const auto& sh : shapes) {
-            std::vector<int> members;
-            for (int i = 0; i < (int)b->probs.size(); ++i)
-                if (b->probs[i].nx == sh[0] && b->probs[i].ny == sh[1] && b->probs[i].nz == sh[2]) members.push_back(i);
-            Engine<Elastic> e(sh[0], sh[1], sh[2], b->precision);
+        for (BatchGroup& G : b->groups) {
+            if (!G.eng) {
+                G.eng = new Engine<Elastic>(G.shape[0], G.shape[1], G.shape[2], b->precision);
+                // RVEs per device batch: 70% of free memory, or the max_solves option
+                size_t fr = 0, tot = 0;
+                HCHK(hipMemGetInfo(&fr, &tot));
+                G.per = (long long)(0.7 * (double)fr / (double)(6 * G.eng->bytes_per_solve() + (size_t)G.eng->N));
+                if (b->max_solves > 0) G.per = std::min<long long>(G.per, b->max_solves / 6);
+                if (G.per < 1) throw ArgError{M3D_ERR_OOM, "batch_homogenization: one RVE's 6 solves do not fit"};
+            }
+            Engine<Elastic>& e = *G.eng;
             e.polish_factor = b->polish_factor;
             e.fp32_floor = b->fp32_floor;
             e.rollback = b->rollback;
             e.cap = b->cap;
             e.chunk = b->chunk;
             e.refine = b->refine;
-            // RVEs per device batch: 70% of free memory, or the max_solves option
-            size_t fr = 0, tot = 0;
-            HCHK(hipMemGetInfo(&fr, &tot));
-            long long per = (long long)(0.7 * (double)fr / (double)(6 * e.bytes_per_solve() + (size_t)e.N));
-            if (b->max_solves > 0) per = std::min<long long>(per, b->max_solves / 6);
-            if (per < 1) throw ArgError{M3D_ERR_OOM, "batch_homogenization: one RVE's 6 solves do not fit"};
-            for (size_t k0 = 0; k0 < members.size(); k0 += (size_t)per) {
-                const size_t k1 = std::min(members.size(), k0 + (size_t)per);
+            const std::vector<int>& members = G.members;
+            for (size_t k0 = 0; k0 < members.size(); k0 += (size_t)G.per) {
+                const size_t k1 = std::min(members.size(), k0 + (size_t)G.per);
                 const int R = (int)(k1 - k0);
-                std::vector<uint8_t> ids((size_t)R * e.N);
-                std::vector<double> coef((size_t)R * 256 * NCOEF), prm((size_t)2 * R);
                 std::vector<std::array<double, 6>> loads;
                 std::vector<double> norms;
                 std::vector<int> sbv, rvv;
                 for (int r = 0; r < R; ++r) {
                     const BatchProblem& q = b->probs[members[k0 + r]];
-                    std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
-                    elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF);
-                    prm[2 * r] = q.P.la0;
-                    prm[2 * r + 1] = q.P.mu0;
                     elastic_unit_loads(q.P, e.N, loads, norms);
                     for (int w = 0; w < 6; ++w) {
                         sbv.push_back(q.P.strain_base);
                         rvv.push_back(r);
                     }
                 }
-                e.upload_material(ids, coef, prm);
+                if (!G.loaded) {
+                    std::vector<uint8_t> ids((size_t)R * e.N);
+                    std::vector<double> coef((size_t)R * 256 * NCOEF), prm((size_t)2 * R);
+                    for (int r = 0; r < R; ++r) {
+                        const BatchProblem& q = b->probs[members[k0 + r]];
+                        std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
+                        elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF);
+                        prm[2 * r] = q.P.la0;
+                        prm[2 * r + 1] = q.P.mu0;
+                    }
+                    e.upload_material(ids, coef, prm);
+                    G.loaded = (R == (int)members.size()); // one device batch: keep it resident
+                }
                 e.solve(loads, tol_field, tol_eq, norms, 1, sbv, rvv);
                 const std::vector<double> m = e.means();
                 for (int r = 0; r < R; ++r) {

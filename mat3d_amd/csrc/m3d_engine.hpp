@@ -471,7 +471,8 @@ struct Engine {
         const dim3 grid(g.nsolve * outer * nkt);
         if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
         const size_t sm = ((size_t)L * TKc * C + L) * sizeof(cx<T>);
-        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>), grid, dim3(fast::XYNT), sm, st, g, p, prm<T>());
+        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>), grid, dim3(fast::xy_threads<KIND, T>()), sm, st, g, p,
+                           prm<T>());
         return true;
     }
 

@@ -74,7 +74,25 @@ constexpr int xytile(int L, int C)
     return tk < 1 ? 1 : tk;
 }
 constexpr int ZNT = 256;
-constexpr int XYNT = 256;
+
+// threads per workgroup of the x/y passes: y pass, x pass with the Green
+// step, x pass of the check FFT
+#ifndef M3D_XY_NT
+#define M3D_XY_NT 512
+#endif
+#ifndef M3D_XG_NT
+#define M3D_XG_NT 512
+#endif
+#ifndef M3D_XC_NT
+#define M3D_XC_NT 256
+#endif
+// (FP64 tiles hold half the lines: they keep 256 threads)
+template <int KIND, typename T>
+constexpr int xy_threads()
+{
+    if (sizeof(T) == 8) return 256;
+    return KIND == 1 ? M3D_XG_NT : (KIND == 2 ? M3D_XC_NT : M3D_XY_NT);
+}
 
 // minimum waves per SIMD the register allocator must leave room for (0: no
 // constraint); these passes are HBM-bound, so occupancy buys latency hiding
@@ -524,11 +542,11 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
 template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK>
-__global__ void __launch_bounds__(XYNT) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+__global__ void __launch_bounds__(xy_threads<KIND, T>()) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
-    constexpr int NT = XYNT;
+    constexpr int NT = xy_threads<KIND, T>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[NT / 64];
     cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
