@@ -542,7 +542,7 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
 template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK>
-__global__ void __launch_bounds__(xy_threads<KIND, T>()) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+__global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
