@@ -85,3 +85,25 @@ def test_j2_uniaxial_closed_form_gpu():
     for eps, s0 in s.prop_homogenized[:, :2]:
         ref = E * eps if s0 <= sy else (eps + sy / H) / (1 / E + 1 / H)
         assert abs(s0 - ref) < 1e-6 * max(abs(ref), sy)
+
+
+def test_damage_uniaxial_closed_form_gpu():
+    """Homogeneous "ed" RVE, uniaxial stress control: d = 1 - exp(-P (sqrt(E) eps - Y0)),
+    sigma = (1 - d) E eps."""
+    n = 32
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    E, nu, Pd, Y0 = 70e9, 0.3, 5e-4, 500.0
+    mat = np.array([[E, nu, np.inf, 0, 1, Pd, Y0], [E, nu, np.inf, 0, 1, Pd, Y0]])
+    s = m.FFT_Analysis3D(n, n, n, "ed", precision="fp64")
+    s.material(mat, ids)
+    X0 = np.eye(6)[0]
+    s.setLoad(0.004 * X0, X0)
+    s.runAnalysis(1e-10, 1e-10, 10)
+    P = s.prop_homogenized
+    assert P.shape == (10, 9)
+    for eps, s0, d in P[:, [0, 1, 7]]:
+        Y = np.sqrt(E) * eps
+        dref = 1 - np.exp(-Pd * (Y - Y0)) if Y > Y0 else 0.0
+        assert abs(d - dref) < 1e-8
+        assert abs(s0 - (1 - dref) * E * eps) < 1e-6 * abs((1 - dref) * E * eps)

@@ -301,3 +301,28 @@ def test_j2_uniaxial_linear_hardening():
         assert abs(s0 - ref) < 1e-6 * max(abs(ref), sy), (eps, s0, ref)
         assert np.abs(row[2:7]).max() < 1e-6 * max(abs(s0), 1.0)  # uniaxial: other components vanish
     assert P[-1, 1] > sy  # the path went plastic
+
+
+def test_damage_uniaxial_closed_form():
+    """Homogeneous "ed" RVE under mixed (uniaxial stress) control along tensor
+    axis 0: Y = sqrt(eps:C eps) = sqrt(E) eps, d = 1 - exp(-P (Y - Y0)) once
+    Y > Y0 (monotone loading), sigma = (1 - d) E eps (FFT_analysis3D.cpp:619-638)."""
+    n = 4
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    E, nu, Pd, Y0 = 70e9, 0.3, 5e-4, 500.0
+    mat = np.array([[E, nu, np.inf, 0, 1, Pd, Y0], [E, nu, np.inf, 0, 1, Pd, Y0]])
+    o = oracle.Elastic(n, n, n, "ed", nthreads=1)
+    o.material(mat, ids)
+    X0 = np.eye(6)[0]
+    o.set_load(0.004 * X0, X0)
+    o.run(1e-10, 1e-10, 10)
+    P = o.prop_homogenized()
+    assert P.shape == (10, 9)
+    for eps, s0, d in P[:, [0, 1, 7]]:
+        Y = np.sqrt(E) * eps
+        dref = 1 - np.exp(-Pd * (Y - Y0)) if Y > Y0 else 0.0
+        assert abs(d - dref) < 1e-8, (eps, d, dref)
+        sref = (1 - dref) * E * eps
+        assert abs(s0 - sref) < 1e-6 * abs(sref), (eps, s0, sref)
+    assert P[-1, 7] > 0.1  # the path went well into damage
