@@ -32,6 +32,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 METRIC = "voxel-iters/s at 256³ (FP32+FP64 polish vs FP64 GPU), C_eff rel err"
 
+# HBM bytes per full-batch launch (6 load cases) of the FP32 kernels on the
+# 256^3 workload, from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000)
+# passes, each scale factor taken from a copy kernel of known size
+# (profiles/elastic256_fibres_hbm_traffic.md)
+MEASURED_TRAFFIC_256 = {
+    "zpass_update": 9.803e9,
+    "ypass_fwd": 5.087e9,
+    "xpass_green": 5.088e9,
+    "ypass_inv": 5.069e9,
+    "zpass_check": 4.952e9,
+    "xpass_check": 2.441e9,
+}
+
 
 def kernel_bytes(geom, r):
     """Algorithmic HBM bytes per launch of each per-iteration kernel: every
@@ -195,7 +208,8 @@ def main():
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
         "c_eff_rel_err_vs_fp64_gpu": c_err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kernels[dom]["GB/s"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": None},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac"],
+                     "traffic": (MEASURED_TRAFFIC_256.get(dom) if (n == 256 and geom["solves"] == 6) else None)},
         "roofline_iteration": {"bytes_per_voxel_iter": round(B, 2), "ms_per_iter_batch": round(it_ms, 4),
                                "achieved": round(it_gbs, 1), "frac": round(it_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
