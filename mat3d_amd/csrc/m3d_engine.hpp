@@ -159,7 +159,7 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
     }
     if (fast_xy(C, Nx)) {
         g.fx = 1;
-        g.TKx = fast::xytile<T>(Nx, C);
+        g.TKx = fast::xtile<T>(Nx, C);
         g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
         g.nbx = Ny * g.nkx;
     }
@@ -465,12 +465,17 @@ struct Engine {
         const int Lax = KIND == 0 ? g.Ny : g.Nx;
         if (Lax != L) return false;
         const int TK = KIND == 0 ? g.TKy : g.TKx;
-        constexpr int TKc = fast::xytile<T>(L, C);
+        constexpr int TKc = KIND == 0 ? fast::xytile<T>(L, C) : fast::xtile<T>(L, C);
         const int nkt = (g.Nzh + TK - 1) / TK;
         const int outer = KIND == 0 ? g.Nx : g.Ny;
         const dim3 grid(g.nsolve * outer * nkt);
         if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
         const size_t sm = ((size_t)L * TKc * C + L) * sizeof(cx<T>);
+        if (sm > 65536) {
+            // gfx950: up to 160 KB of LDS per workgroup
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        }
         hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>), grid, dim3(fast::xy_threads<KIND, T>()), sm, st, g, p,
                            prm<T>());
         return true;

@@ -34,25 +34,49 @@ using stage_t = typename std::conditional<PAIR, vf4, typename std::conditional<s
 constexpr int n_twos(int L) { return (L % 2 == 0) ? 1 + n_twos(L / 2) : 0; }
 constexpr int odd_part(int L) { return (L % 2 == 0) ? odd_part(L / 2) : L; }
 constexpr int n_threes(int L) { return (L % 3 == 0) ? 1 + n_threes(L / 3) : 0; }
-// radix sequence: 8s, then 4*4 instead of 8*2, a 4 or a 2, then 3s (same
-// policy as the runtime planner)
+// radix sequence of the compile-time passes: 16s for the powers of two (a
+// leftover factor 8 or 4 as one more pass; 2 as 16*2 -> 8*4), then 3s.  Fewer
+// passes mean fewer LDS round trips per line.
+#ifndef M3D_RADIX16
+#define M3D_RADIX16 1
+#endif
 constexpr int n8(int e) { return e >= 3 ? ((e == 4) ? 0 : 1 + n8(e - 3)) : 0; }
 constexpr int rest2(int e) { return e >= 3 ? ((e == 4) ? 4 : rest2(e - 3)) : e; }
-constexpr int npass(int L)
+// powers-of-two radices of 2^e, pass p (0 if p is past them)
+constexpr int pow2_radix(int e, int p)
 {
-    int e = n_twos(L), r = rest2(e);
-    int p = n8(e) + (r == 4 ? 2 : (r == 2 ? 1 : (r == 1 ? 1 : 0)));
-    return p + n_threes(L);
+    if (!M3D_RADIX16) {
+        const int a = n8(e), r = rest2(e);
+        if (p < a) return 8;
+        p -= a;
+        if (r == 4) return p < 2 ? 4 : 0;
+        if (r == 2) return p < 1 ? 4 : 0;
+        if (r == 1) return p < 1 ? 2 : 0;
+        return 0;
+    }
+    const int n16 = e / 4, rem = e % 4;
+    if (rem == 1 && n16 >= 1) {
+        // 16^(n16-1) * 8 * 4
+        if (p < n16 - 1) return 16;
+        p -= n16 - 1;
+        return p == 0 ? 8 : (p == 1 ? 4 : 0);
+    }
+    if (p < n16) return 16;
+    p -= n16;
+    if (p == 0 && rem > 0) return 1 << rem;
+    return 0;
 }
+constexpr int pow2_passes(int e)
+{
+    int n = 0;
+    while (pow2_radix(e, n) != 0) ++n;
+    return n;
+}
+constexpr int npass(int L) { return pow2_passes(n_twos(L)) + n_threes(L); }
 constexpr int radix(int L, int p)
 {
-    int e = n_twos(L), a = n8(e), r = rest2(e);
-    if (p < a) return 8;
-    p -= a;
-    if (r == 4) { if (p < 2) return 4; p -= 2; }
-    else if (r == 2) { if (p < 1) return 4; p -= 1; }
-    else if (r == 1) { if (p < 1) return 2; p -= 1; }
-    return 3;
+    const int e = n_twos(L), a = pow2_passes(e);
+    return p < a ? pow2_radix(e, p) : 3;
 }
 constexpr bool supported(int L) { return odd_part(L) == 1 || odd_part(L) == 3 || odd_part(L) == 9; }
 
@@ -70,6 +94,18 @@ template <typename T>
 constexpr int xytile(int L, int C)
 {
     int budget = sizeof(T) == 4 ? 6144 : 3072;
+    int tk = budget / (C * L);
+    return tk < 1 ? 1 : tk;
+}
+// x passes: complex values per tile (FP32; FP64 half); the x lines are one
+// (y, kz) row apart, Ny*Nzh*C values, so wider kz tiles amortise each row visit
+#ifndef M3D_XTILE
+#define M3D_XTILE 6144
+#endif
+template <typename T>
+constexpr int xtile(int L, int C)
+{
+    int budget = sizeof(T) == 4 ? M3D_XTILE : M3D_XTILE / 2;
     int tk = budget / (C * L);
     return tk < 1 ? 1 : tk;
 }

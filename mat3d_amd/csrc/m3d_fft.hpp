@@ -113,6 +113,43 @@ __device__ __forceinline__ void butterfly(cx<T> (&v)[R], const cx<T>* __restrict
         v[6] = e[2] - o2;
         v[3] = e[3] + o3;
         v[7] = e[3] - o3;
+    } else if constexpr (R == 16) {
+        // 16 = 4 x 4: 4-point DFTs over n1 of x[4 n1 + n2], twiddles
+        // W16^(n2 k1), 4-point DFTs over n2; X[k1 + 4 k2] lands in v[k1 + 4 k2]
+        const T c1 = T(0.92387953251128675612818318939679); // cos(pi/8)
+        const T s1 = T(0.38268343236508977172845998403040); // sin(pi/8)
+        const T h = T(0.70710678118654752440084436210485);
+        cx<T> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            const cx<T> a = v[n2], b = v[n2 + 4], c = v[n2 + 8], d = v[n2 + 12];
+            const cx<T> s02 = a + c, d02 = a - c, s13 = b + d, d13 = rot90<DIR>(b - d);
+            y[n2][0] = s02 + s13;
+            y[n2][1] = d02 + d13;
+            y[n2][2] = s02 - s13;
+            y[n2][3] = d02 - d13;
+        }
+        // W16^m = (cos, DIR*sin)(2 pi m / 16): forward exp(-i ...), inverse exp(+i ...)
+        auto tmul = [](cx<T> z, T wr, T wi) { return cx<T>{z.re * wr - z.im * wi, z.re * wi + z.im * wr}; };
+        const T sd = DIR > 0 ? T(1) : T(-1);
+        y[1][1] = tmul(y[1][1], c1, sd * s1);   // m = 1
+        y[1][2] = tmul(y[1][2], h, sd * h);     // m = 2
+        y[1][3] = tmul(y[1][3], s1, sd * c1);   // m = 3
+        y[2][1] = tmul(y[2][1], h, sd * h);     // m = 2
+        y[2][2] = rot90<DIR>(y[2][2]);          // m = 4: -i forward, +i inverse
+        y[2][3] = tmul(y[2][3], -h, sd * h);    // m = 6
+        y[3][1] = tmul(y[3][1], s1, sd * c1);   // m = 3
+        y[3][2] = tmul(y[3][2], -h, sd * h);    // m = 6
+        y[3][3] = tmul(y[3][3], -c1, -sd * s1); // m = 9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            const cx<T> a = y[0][k1], b = y[1][k1], c = y[2][k1], d = y[3][k1];
+            const cx<T> s02 = a + c, d02 = a - c, s13 = b + d, d13 = rot90<DIR>(b - d);
+            v[k1] = s02 + s13;
+            v[k1 + 4] = d02 + d13;
+            v[k1 + 8] = s02 - s13;
+            v[k1 + 12] = d02 - d13;
+        }
     } else {
         // generic odd radix: direct DFT, W_R^m = tw[m * wstep]
         cx<T> out[R];
