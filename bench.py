@@ -100,7 +100,7 @@ def main():
                     help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
                          "reference for HBM byte counters, then exit")
     ap.add_argument("--workload", default="fibres256",
-                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192"],
+                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64"],
                     help="fibres256 is the headline line; the others measure the batch, thermal and "
                          "J2 paths on their own workloads")
     args = ap.parse_args()
@@ -326,7 +326,36 @@ def bench_j2_192(args):
           {"iterations": info["iters_fp64"], "final_sigma11": P[-1, 1]})
 
 
-OTHER = {"voronoi_batch": bench_voronoi_batch, "thermal512": bench_thermal512, "j2_192": bench_j2_192}
+def bench_basic64(args):
+    """64^3 two-phase elastic RVE (matrix E=70 GPa nu=0.30, spheres r=6 at 20%,
+    E=400 GPa nu=0.20, seed 1), basic Moulinec-Suquet scheme with the 6 unit
+    strain loads held fixed, arithmetic-mean reference medium, FP64, tol 1e-6."""
+    import mat3d_amd as m
+    from mat3d_amd import gen
+
+    n = 64
+    ids = gen.c1_rve(n)
+    s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
+    s.set_option("scheme", 1)
+    s.material(gen.MAT_C1, ids)
+    for i in range(args.warmup):
+        s.homogenization(1e-6, 1e-6)
+    vi = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s.homogenization(1e-6, 1e-6)
+        vi += n**3 * sum(d["iterations"] for d in s.info)
+    dt = time.perf_counter() - t0
+    its = [d["iterations"] for d in s.info]
+    print(f"[bench] basic64: {vi / dt:.4e} voxel-it/s, {dt / args.steps * 1e3:.1f} ms/step, iterations {its}, "
+          f"C11 {s.CHomo[0, 0]:.6e}", flush=True)
+    _line(args, "voxel-iters/s, 64^3 basic scheme FP64 (6 strain loads)", vi / dt, "voxel-iters/s",
+          dt / args.steps * 1e3, "f64", "64^3 spheres r=6 20% (E 400/70 GPa), basic scheme, fixed strain loads, tol 1e-6",
+          {"iterations": its, "C_eff": s.CHomo.tolist()})
+
+
+OTHER = {"voronoi_batch": bench_voronoi_batch, "thermal512": bench_thermal512, "j2_192": bench_j2_192,
+         "basic64": bench_basic64}
 
 
 if __name__ == "__main__":

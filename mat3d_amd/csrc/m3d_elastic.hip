@@ -11,6 +11,7 @@ struct m3d_elastic {
     std::string err;
     std::string behav;
     bool have_mat = false, have_load = false, ran = false;
+    int scheme = 0; // 0 accelerated (reference "e" loop), 1 basic with the load held fixed
     ElasticProps P;
     double E[6] = {0, 0, 0, 0, 0, 0}, X0[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -42,7 +43,7 @@ struct m3d_batch {
     bool grouped = false;
     bool ran = false;
     double polish_factor = 10.0, fp32_floor = 1e-6;
-    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1;
+    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1, scheme = 0;
     double ms = 0, voxel_its = 0;
 };
 
@@ -126,7 +127,14 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
         else if (k == "cap" && value >= 1) h->eng->cap = (int)value;
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
         else if (k == "refine" && (value == 0 || value == 1)) h->eng->refine = (int)value;
-        else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
+        else if (k == "scheme" && (value == 0 || value == 1)) {
+            h->scheme = (int)value;
+            if (h->have_mat) {
+                std::vector<double> coef((size_t)256 * NCOEF);
+                elastic_tables(h->P, coef.data(), h->scheme == 1);
+                h->eng->upload_coef(coef, {h->P.la0, h->P.mu0});
+            }
+        } else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
     });
 }
@@ -140,7 +148,7 @@ int m3d_elastic_material(m3d_elastic* h, const double* mat, const int32_t* rve, 
         ElasticProps P;
         elastic_props(h->behav, mat, rve, nph, e.N, P, ids.data());
         std::vector<double> coef((size_t)256 * NCOEF);
-        elastic_tables(P, coef.data());
+        elastic_tables(P, coef.data(), h->scheme == 1);
         e.upload_material(ids, coef, {P.la0, P.mu0});
         h->P = std::move(P);
         h->have_mat = true;
@@ -312,6 +320,10 @@ int m3d_batch_set_option(m3d_batch* b, const char* key, double value)
             batch_free_groups(b); // device batch sizes change
         }
         else if (k == "refine" && (value == 0 || value == 1)) b->refine = (int)value;
+        else if (k == "scheme" && (value == 0 || value == 1)) {
+            b->scheme = (int)value;
+            for (BatchGroup& G : b->groups) G.loaded = false; // coefficient tables change
+        }
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
     });
@@ -402,7 +414,7 @@ int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
                     for (int r = 0; r < R; ++r) {
                         const BatchProblem& q = b->probs[members[k0 + r]];
                         std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
-                        elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF);
+                        elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF, b->scheme == 1);
                         prm[2 * r] = q.P.la0;
                         prm[2 * r + 1] = q.P.mu0;
                     }

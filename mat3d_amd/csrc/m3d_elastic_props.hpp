@@ -76,7 +76,9 @@ void elastic_props(const std::string& behav, const double* mat, const int32_t* r
 // Per-phase coefficients of the isotropic maps used by the kernels
 // (m3d_kernels.hpp, Elastic policy): stiffness or compliance, polarisation
 // and the Eyre-Milton acceleration C_acc = 2(C+C0)^-1 C0 (:539-546).
-void elastic_tables(const ElasticProps& P, double* coef)
+// basic: the basic Moulinec-Suquet iteration with the load held fixed,
+// u <- u_c (acceleration factors 1)
+void elastic_tables(const ElasticProps& P, double* coef, bool basic = false)
 {
     const double K0 = P.la0 + 2.0 * P.mu0 / 3.0, mu0 = P.mu0;
     std::fill(coef, coef + (size_t)256 * NCOEF, 0.0);
@@ -93,6 +95,7 @@ void elastic_tables(const ElasticProps& P, double* coef)
             c[3] = 1.0 / (2 * mu) - 1.0 / (2 * mu0);
             c[4] = 2 * K / (K + K0);    c[5] = 2 * mu / (mu + mu0);
         }
+        if (basic) c[4] = c[5] = 1.0;
     }
 }
 

@@ -879,6 +879,18 @@ struct Engine {
         std::memcpy(out, v, sizeof v);
     }
 
+    // replace the coefficient tables and reference media (same layout as
+    // upload_material), keeping the phase ids
+    void upload_coef(const std::vector<double>& coef, const std::vector<double>& prmv)
+    {
+        std::vector<float> c32(coef.begin(), coef.end());
+        std::vector<float> p32(prmv.begin(), prmv.end());
+        HCHK(hipMemcpy(coef32, c32.data(), sizeof(float) * c32.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(coef64, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(prm32, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(prm64, prmv.data(), sizeof(double) * prmv.size(), hipMemcpyHostToDevice));
+    }
+
     // upload phase ids (validated, < 256 phases) and coefficient tables
     // ids: N phase ids per RVE; coef: 256*NCOEF per RVE; prmv: 2 per RVE
     void upload_material(const std::vector<uint8_t>& ids, const std::vector<double>& coef, const std::vector<double>& prmv)

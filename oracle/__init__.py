@@ -53,6 +53,7 @@ def lib():
             "m3o_el_create": (vp, [ip, ip, ip, ip, ip]),
             "m3o_el_destroy": (None, [vp]),
             "m3o_el_set_cap": (None, [vp, ip]),
+            "m3o_el_set_scheme": (None, [vp, ip]),
             "m3o_el_material": (ip, [vp, dp, i32p, ip]),
             "m3o_el_set_load": (None, [vp, dp, dp]),
             "m3o_el_run": (ip, [vp, ct.c_double, ct.c_double, ip]),
@@ -146,6 +147,10 @@ class Elastic:
 
     def set_cap(self, cap):
         lib().m3o_el_set_cap(self._h, int(cap))
+
+    def set_scheme(self, basic):
+        """1: basic scheme with the load held fixed (update eps <- eps_c)."""
+        lib().m3o_el_set_scheme(self._h, int(basic))
 
     def material(self, mat, rve, nph=None):
         mat = np.asarray(mat, dtype=np.float64)

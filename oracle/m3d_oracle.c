@@ -385,6 +385,7 @@ typedef struct {
     double *C, *S, *Cacc, *Sacc;  /* nph x 36 */
     double C0[36], S0[36], la0, mu0;
     int strainBase, stressBase;
+    int basic;                    /* 1: basic scheme in the "e" loop (update with C_acc = I) */
     int32_t *rve;
     double *eps, *sig;            /* 6 x N, column-major (voxel-major AoS) */
     double *t1;
@@ -467,6 +468,13 @@ void m3o_el_destroy(m3o_el *s)
 }
 
 void m3o_el_set_cap(m3o_el *s, int cap) { s->cap = cap; }
+
+/* Basic Moulinec-Suquet scheme with the load held fixed (strain control): the
+ * "e" loop with the update replaced by eps <- eps_c (sigma <- sigma_c in the
+ * stress base), i.e. C_acc = I.  The reference runs its basic scheme only
+ * with the updateE mixed control (FFT_analysis3D.cpp:796-853); this is the
+ * plain fixed-E variant of the same iteration. */
+void m3o_el_set_scheme(m3o_el *s, int basic) { s->basic = basic ? 1 : 0; }
 
 /* setGF, FFT_analysis3D.cpp:155-233: 21 unique Gamma0 (strain base) or Delta0
  * (stress base) components per half-spectrum frequency. */
@@ -622,6 +630,11 @@ static void el_update(m3o_el *s)
 #pragma omp parallel for num_threads(s->nthreads)
     for (long i = 0; i < s->N; ++i) {
         double dlt6[6], y[6];
+        if (s->basic) {
+            if (s->strainBase) for (int c = 0; c < 6; ++c) s->eps[6 * i + c] = s->sig[6 * i + c];
+            else for (int c = 0; c < 6; ++c) s->sig[6 * i + c] = s->eps[6 * i + c];
+            continue;
+        }
         if (s->strainBase) {
             for (int c = 0; c < 6; ++c) dlt6[c] = s->sig[6 * i + c] - s->eps[6 * i + c];
             gemv6(s->Cacc + 36 * s->rve[i], dlt6, y);

@@ -191,6 +191,34 @@ def test_anchored_polish_matches_fp64_polish(case):
     assert field_rel(out[1][2], out[0][2]) < 1e-5
 
 
+@pytest.mark.parametrize("shape,precision", [((64, 64, 64), "fp64"), ((12, 16, 10), "fp64"), ((32, 32, 32), "fp32_polish")])
+def test_basic_scheme_fixed_load_matches_oracle(shape, precision):
+    """Basic Moulinec-Suquet scheme with the unit loads held fixed ("ep"
+    reference medium: arithmetic means, strain base), scheme option 1."""
+    n = max(shape)
+    ids = gen.spheres(n, 6 if n >= 64 else 3, 0.2, seed=1)[: shape[0], : shape[1], : shape[2]].copy()
+    mat = gen.MAT_C1
+    o = oracle.Elastic(*shape, "ep")
+    o.set_scheme(1)
+    o.material(mat, ids)
+    r = o.homogenization(1e-6, 1e-6)
+    s = m.FFT_Analysis3D(*shape, "ep", precision=precision)
+    s.set_option("scheme", 1)
+    s.material(mat, ids)
+    s.homogenization(1e-6, 1e-6)
+    its = [i["iterations"] for i in s.info]
+    err = rel(s.CHomo, r["C"])
+    print(f"[basic {shape} {precision}] iters {its} oracle {r['iters']} C rel err {err:.3e}")
+    if precision == "fp64":
+        assert its == r["iters"]
+        assert err < 1e-9
+    else:
+        assert err < 1e-6
+    _, sig_o = o.fields()
+    _, sig = s.fields(5)
+    assert field_rel(sig, sig_o) < (1e-9 if precision == "fp64" else 1e-5)
+
+
 # ---------------------------------------------------------------------------
 # compile-time specialised kernels: fixed iteration counts against the oracle
 def _fixed_iters_elastic(shape, ids, mat, w, k, precision):

@@ -326,3 +326,46 @@ def test_damage_uniaxial_closed_form():
         sref = (1 - dref) * E * eps
         assert abs(s0 - sref) < 1e-6 * abs(sref), (eps, s0, sref)
     assert P[-1, 7] > 0.1  # the path went well into damage
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_basic_scheme_laminate_exact(axis):
+    """Basic scheme with the load held fixed ("ep" medium: arithmetic means,
+    strain base) converges to the exact laminate solution."""
+    n = 12
+    layers = 5
+    ids = np.ones((n, n, n), dtype=np.int32)
+    sl = [slice(None)] * 3
+    sl[2 - axis] = slice(0, layers)
+    ids[tuple(sl)] = 0
+    mat = np.array([[300e9, 0.25, np.inf, 0, 1, 0, 0], [30e9, 0.35, np.inf, 0, 1, 0, 0]])
+    o = oracle.Elastic(n, n, n, "ep", nthreads=4)
+    o.set_scheme(1)
+    o.material(mat, ids)
+    assert o.info()["strain_base"]
+    r = o.homogenization(1e-12, 1e-12)
+    f0 = layers / n
+    ep = np.zeros((6, 6)); sg = np.zeros((6, 6))
+    for w in range(6):
+        E = np.zeros(6); E[w] = 0.0005
+        e, s = laminate_exact(mat[:, :2], [f0, 1 - f0], axis, E, True)
+        ep[:, w] = e
+        sg[w, :] = s
+    Cref = np.linalg.inv(ep) @ sg
+    assert np.abs(r["C"] - Cref).max() < 1e-6 * np.abs(Cref).max()
+
+
+def test_basic_and_accelerated_schemes_share_the_fixed_point():
+    """Same RVE, same reference medium: the basic scheme (C_acc = I) needs more
+    iterations but converges to the accelerated scheme's C_eff."""
+    n = 16
+    ids = gen.spheres(n, 3, 0.2, seed=3)
+    out = {}
+    for basic in (0, 1):
+        o = oracle.Elastic(n, n, n, "ep", nthreads=4)
+        o.set_scheme(basic)
+        o.material(gen.MAT_C1, ids)
+        out[basic] = o.homogenization(1e-10, 1e-10)
+    rel = np.linalg.norm(out[1]["C"] - out[0]["C"]) / np.linalg.norm(out[0]["C"])
+    assert rel < 1e-8
+    assert min(out[1]["iters"]) > max(out[0]["iters"])
