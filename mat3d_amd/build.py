@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("m3d_elastic.hip", "m3d_thermal.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("m3d_elastic.hip", "m3d_thermal.hip", "m3d_basic.hip")]
 LIB = os.path.join(HERE, "libmat3d.so")
 
 HIPCC_FLAGS = [
@@ -23,7 +23,7 @@ HIPCC_FLAGS = [
 
 
 def build(verbose: bool = False, out: str | None = None, defines: list[str] | None = None) -> str:
-    """Compile the two ABI units in parallel, then link libmat3d.so (or
+    """Compile the three units in parallel, then link libmat3d.so (or
     ``out``); ``defines`` are extra -D macros (tuning parameters)."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     lib = out or LIB

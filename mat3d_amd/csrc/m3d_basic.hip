@@ -1,0 +1,369 @@
+// m3d_basic.hip -- compile-time specialised kernels of the basic scheme with
+// mixed control and the J2 / damage updates (runAnalysis "ep" / "ed",
+// src/3d/FFT_analysis3D.cpp:796-853), FP64.  The host driver is BasicRunner
+// (m3d_basic_host.hpp); it calls the launchers at the end of this file for
+// grids whose line lengths have specialised kernels and the runtime-planned
+// kernels of m3d_basic.hpp otherwise.
+#include "m3d_basic_fast.hpp"
+
+#include <stdexcept>
+
+namespace m3d {
+namespace bfast {
+
+// component-count tags for the y pass
+struct Comp12 {
+    static constexpr int C = 12;
+};
+struct Comp6 {
+    static constexpr int C = 6;
+};
+
+// ------------------------------------------------------------------ z pass
+// Block: plane x, lines y0..y0+TY-1.  LDS row k (k = 0..LZ) holds W = 12 TY
+// columns, column yl*12 + c: c < 6 sigma, c >= 6 eps.  Slot m of the
+// half-length complex FFT holds voxels (2m, 2m+1) of its line after the c2r.
+//   ITER : eps^ (Green output, 6 comps) -> c2r -> eps; START: eps from HBM
+//   then per voxel the constitutive update (elastoPlastic :640-691 or
+//   elastoDamage :619-638) -> sigma; FP64 partial sums of sigma and eps
+//   (updateE :583-594); r2c of the 12 lines -> [sigma^ | eps^]
+template <int BEH, int MODE, int LZ, int TY>
+__global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<double> p)
+{
+    using T = double;
+    constexpr int C = 12;
+    constexpr int W = TY * C;
+    constexpr int NZ = 2 * LZ;
+    constexpr int NZH = LZ + 1;
+    constexpr int NT = ZNT;
+    constexpr int NW = NT / 64;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NBSUM][NW];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = sl + NZH * W;
+
+    const int per = g.Nx * (g.Ny / TY);
+    const int t = fast::xcd_remap(blockIdx.x, gridDim.x);
+    const int x = t / (g.Ny / TY);
+    const int y0 = (t - x * (g.Ny / TY)) * TY;
+    Ctl& ctl = p.ctl[0];
+    if (ctl.done || t >= per) return;
+    const int tid = threadIdx.x;
+    const long long vox0 = ((long long)x * g.Ny + y0) * NZ;
+    T* eps = p.eps + vox0 * 6;
+    T* sig = p.sig + vox0 * 6;
+    const uint8_t* ph = p.ph + vox0;
+    for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
+
+    if (MODE == BMODE_ITER) {
+        const cx<T>* src = p.S6 + ((long long)x * g.Ny + y0) * NZH * 6;
+        for (int e = tid; e < TY * NZH * 6; e += NT) {
+            const int yl = e / (NZH * 6);
+            const int r = e - yl * (NZH * 6);
+            const int kz = r / 6, c = r - kz * 6;
+            sl[kz * W + yl * C + 6 + c] = src[e];
+        }
+        __syncthreads();
+        // c2r pre-processing of the eps lines; kz = 0 and Nz/2 keep real parts
+        constexpr int half = LZ / 2 + 1;
+        constexpr int W6 = TY * 6;
+        for (int e = tid; e < W6 * half; e += NT) {
+            const int li = e % W6, k = e / W6;
+            const int l = (li / 6) * C + 6 + li % 6;
+            if (k == 0) {
+                const T x0 = sl[l].re, xl = sl[LZ * W + l].re;
+                sl[l] = cx<T>{x0 + xl, x0 - xl};
+            } else {
+                const int kk = LZ - k;
+                const cx<T> Xk = sl[k * W + l], Xc = sl[kk * W + l];
+                cx<T> w = tw[k];
+                w.im = -w.im;
+                sl[k * W + l] = (Xk + cconj(Xc)) + rot90<1>(cmul(w, Xk - cconj(Xc)));
+                if (kk != k) {
+                    cx<T> w2 = tw[kk];
+                    w2.im = -w2.im;
+                    sl[kk * W + l] = (Xc + cconj(Xk)) + rot90<1>(cmul(w2, Xc - cconj(Xk)));
+                }
+            }
+        }
+        __syncthreads();
+        fast::fft_sub<T, LZ, W6, NT, 1, 2, W, 6, C, 6>(sl, tw);
+    } else {
+        __syncthreads();
+    }
+
+    // constitutive update per voxel pair
+    double acc[NBSUM];
+#pragma unroll
+    for (int q = 0; q < NBSUM; ++q) acc[q] = 0.0;
+    bool bad = false;
+    for (int q = tid; q < TY * LZ; q += NT) {
+        const int yl = q / LZ, m = q - yl * LZ;
+        const int v0 = yl * NZ + 2 * m;
+        T e2[12], s2[12];
+        if (MODE == BMODE_ITER) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const cx<T> u = sl[m * W + yl * C + 6 + c];
+                e2[c] = u.re;
+                e2[6 + c] = u.im;
+            }
+            fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
+        } else {
+            fast::vload<T, 12>(eps + (long long)v0 * 6, e2);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const long long vg = vox0 + v0 + h;
+            const double* mt = p.mat + ph[v0 + h] * 8;
+            T* e = e2 + 6 * h;
+            T* sg = s2 + 6 * h;
+            if (BEH == BEH_EP) {
+                T so[6], eo[6], phi, dep, seff;
+                fast::vload<T, 6>(p.sig_old + vg * 6, so);
+                fast::vload<T, 6>(p.eps_old + vg * 6, eo);
+                if (!j2_update<T>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+                p.phi[vg] = phi;
+                p.dep[vg] = dep;
+                p.seff[vg] = seff;
+            } else {
+                T d = p.ep[vg];
+                damage_update<T>(mt, e, d, sg);
+                p.ep[vg] = d;
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                acc[c] += (double)sg[c];
+                acc[6 + c] += (double)e[c];
+            }
+        }
+        fast::vstore<T, 12>(sig + (long long)v0 * 6, s2);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            sl[m * W + yl * C + c] = cx<T>{s2[c], s2[6 + c]};
+            if (MODE == BMODE_START) sl[m * W + yl * C + 6 + c] = cx<T>{e2[c], e2[6 + c]};
+        }
+    }
+    if (bad) ctl.err = 1;
+    // fixed-order block sums of the 12 means
+    {
+        const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+        for (int q = 0; q < NBSUM; ++q) {
+            const double v = wave_sum(acc[q]);
+            if (lane == 0) red[q][wid] = v;
+        }
+        __syncthreads();
+        if (tid < NBSUM) {
+            double r = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) r += red[tid][w];
+            p.psum[(long long)tid * g.nbz + t] = r;
+        }
+    }
+    __syncthreads();
+
+    // r2c of the 12 lines
+    fast::fft<T, LZ, W, NT, -1, 2>(sl, tw);
+    {
+        constexpr int half = LZ / 2 + 1;
+        for (int e = tid; e < W * half; e += NT) {
+            const int l = e % W, k = e / W;
+            if (k == 0) {
+                const cx<T> z0 = sl[l];
+                sl[l] = cx<T>{z0.re + z0.im, T(0)};
+                sl[LZ * W + l] = cx<T>{z0.re - z0.im, T(0)};
+            } else {
+                const int kk = LZ - k;
+                const cx<T> zk = sl[k * W + l], zc = cconj(sl[kk * W + l]);
+                const cx<T> E = cscale(zk + zc, T(0.5));
+                const cx<T> O = cscale(rot90<-1>(zk - zc), T(0.5));
+                sl[k * W + l] = E + cmul(tw[k], O);
+                if (kk != k) sl[kk * W + l] = cconj(E) + cmul(tw[kk], cconj(O));
+            }
+        }
+    }
+    __syncthreads();
+    cx<T>* dst = p.S12 + ((long long)x * g.Ny + y0) * NZH * C;
+    for (int e = tid; e < TY * NZH * C; e += NT) {
+        const int yl = e / (NZH * C);
+        const int r = e - yl * (NZH * C);
+        const int kz = r / C, c = r - kz * C;
+        dst[e] = sl[kz * W + yl * C + c];
+    }
+}
+
+// ------------------------------------------------------------------ x pass
+// Block: row y, kz tile [kz0, kz0+nk); LDS row kx holds W = 12 TK columns,
+// column kzl*12 + c.  Forward x FFT of the 12 lines, eqVar partials from
+// sigma^ (convergenceCheck :351-389), Green step t = eps^ - Gamma0:sigma^
+// (multiplyGF :323-338, DC = E) written over the sigma columns, inverse x
+// FFT of those 6 columns per kz.
+template <int L, int TK>
+__global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<double> p)
+{
+    using T = double;
+    constexpr int C = 12;
+    constexpr int W = TK * C;
+    constexpr int NT = XNT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NT / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = sl + L * W;
+
+    const int nkt = (g.Nzh + TK - 1) / TK;
+    const int t = fast::xcd_remap(blockIdx.x, gridDim.x);
+    const int y = t / nkt;
+    const int kz0 = (t - y * nkt) * TK;
+    const Ctl& ctl = p.ctl[0];
+    if (ctl.done || y >= g.Ny) return;
+    const int nk = min(TK, g.Nzh - kz0);
+    const int tid = threadIdx.x;
+    const cx<T>* base = p.S12 + ((long long)y * g.Nzh + kz0) * C;
+    const long long xstride = (long long)g.Ny * g.Nzh * C;
+    for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
+    fast::rows_to_lds<T, L, W, NT, false>(sl, base, xstride, nk * C);
+    __syncthreads();
+    fast::fft<T, L, W, NT, -1, 1>(sl, tw);
+
+    const T la0 = T(ctl.la0), mu0 = T(ctl.mu0);
+    double acc = 0.0;
+    for (int f = tid; f < L * nk; f += NT) {
+        const int kx = f / nk, kzl = f - kx * nk;
+        const int kz = kz0 + kzl;
+        cx<T>* v = sl + kx * W + kzl * C;
+        double a2[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a2[c] = (double)v[c].re * (double)v[c].re + (double)v[c].im * (double)v[c].im;
+        if (kz >= 1) {
+            const double z0 = Elastic::fmap(kz, g.Nz), z1 = Elastic::fmap(y, g.Ny), z2 = Elastic::fmap(kx, g.Nx);
+            const double q = a2[0] * z0 * z0 + a2[1] * z1 * z1 + a2[2] * z2 * z2 + a2[3] * ((z1 + z1) + (z2 + z2)) +
+                             a2[4] * ((z0 + z0) + (z2 + z2)) + a2[5] * ((z0 + z0) + (z1 + z1));
+            acc += sqrt(fabs(q));
+        }
+        if (kx == 0 && y == 0 && kz == 0) {
+            p.pchk[g.nbx] = sqrt(a2[0] + a2[1] + a2[2] + 2 * a2[3] + 2 * a2[4] + 2 * a2[5]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
+        } else {
+            const int fz = Elastic::fmap(kz, g.Nz), fy = Elastic::fmap(y, g.Ny), fxx = Elastic::fmap(kx, g.Nx);
+            T xi[3] = {T(fz), T(fy), T(fxx)};
+            cx<T> sg[6], o[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sg[c] = v[c];
+            green_elastic<T>(xi, sg, o, 1, la0, mu0);
+            // kz = 0 and Nz/2 planes: Hermitian part of the Gamma0 term
+            if (kz == 0 || kz == g.Nz / 2) {
+                const int px = (g.Nx - kx) % g.Nx, py = (g.Ny - y) % g.Ny;
+                const int gz = Elastic::fmap(kz, g.Nz), gy = Elastic::fmap(py, g.Ny), gx = Elastic::fmap(px, g.Nx);
+                if (gz != -fz || gy != -fy || gx != -fxx) {
+                    T xp[3] = {T(gz), T(gy), T(gx)};
+                    cx<T> o2[6];
+                    green_elastic<T>(xp, sg, o2, 1, la0, mu0);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) o[c] = cscale(o[c] + o2[c], T(0.5));
+                }
+            }
+            const T sc = T(g.invN);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[c] = cscale(v[6 + c] - o[c], sc);
+        }
+    }
+    const double bs = block_sum(acc, red);
+    if (tid == 0) p.pchk[t] = bs;
+    __syncthreads();
+    fast::fft_sub<T, L, TK * 6, NT, 1, 1, W, 6, C, 0>(sl, tw);
+    cx<T>* ob = p.S6 + ((long long)y * g.Nzh + kz0) * 6;
+    const long long ostride = (long long)g.Ny * g.Nzh * 6;
+    for (int e = tid; e < L * nk * 6; e += NT) {
+        const int xx = e / (nk * 6), r = e - xx * (nk * 6);
+        const int kzl = r / 6, c = r - kzl * 6;
+        ob[xx * ostride + r] = sl[xx * W + kzl * C + c];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+namespace {
+
+template <int BEH, int MODE, int LZ>
+bool z_at(const Geo& g, const BPtr<double>& p, hipStream_t st)
+{
+    if (g.Nz != 2 * LZ) return false;
+    constexpr int TY = zlines(LZ, 2 * LZ);
+    if (g.TY != TY) throw std::runtime_error("basic z tile mismatch");
+    const size_t sm = ((size_t)(LZ + 1) * TY * 12 + 2 * LZ) * sizeof(cx<double>);
+    hipLaunchKernelGGL((kzb<BEH, MODE, LZ, TY>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
+    return true;
+}
+
+template <int BEH, int MODE>
+void z_mode(const Geo& g, const BPtr<double>& p, hipStream_t st)
+{
+    const bool ok = z_at<BEH, MODE, 128>(g, p, st) || z_at<BEH, MODE, 96>(g, p, st) || z_at<BEH, MODE, 64>(g, p, st) ||
+                    z_at<BEH, MODE, 48>(g, p, st) || z_at<BEH, MODE, 32>(g, p, st) || z_at<BEH, MODE, 24>(g, p, st) ||
+                    z_at<BEH, MODE, 16>(g, p, st);
+    if (!ok) throw std::runtime_error("no basic z kernel for this grid");
+}
+
+template <typename PH, int DIR, int WHICH, int L>
+bool y_at(const Geo& g, const Ptr<double>& p, hipStream_t st)
+{
+    if (g.Ny != L) return false;
+    constexpr int TK = fast::xytile<double>(L, PH::C);
+    const int nkt = (g.Nzh + TK - 1) / TK;
+    const size_t sm = ((size_t)L * TK * PH::C + L) * sizeof(cx<double>);
+    hipLaunchKernelGGL((fast::kxy<double, PH, 0, DIR, WHICH, L, TK>), dim3(g.Nx * nkt),
+                       dim3(fast::xy_threads<0, double>()), sm, st, g, p, (const double*)nullptr);
+    return true;
+}
+
+template <typename PH, int DIR, int WHICH>
+void y_run(const Geo& g, const Ptr<double>& p, hipStream_t st)
+{
+    const bool ok = y_at<PH, DIR, WHICH, 256>(g, p, st) || y_at<PH, DIR, WHICH, 192>(g, p, st) ||
+                    y_at<PH, DIR, WHICH, 128>(g, p, st) || y_at<PH, DIR, WHICH, 96>(g, p, st) ||
+                    y_at<PH, DIR, WHICH, 64>(g, p, st) || y_at<PH, DIR, WHICH, 48>(g, p, st) ||
+                    y_at<PH, DIR, WHICH, 32>(g, p, st);
+    if (!ok) throw std::runtime_error("no basic y kernel for this grid");
+}
+
+template <int L>
+bool x_at(const Geo& g, const BPtr<double>& p, hipStream_t st)
+{
+    if (g.Nx != L) return false;
+    constexpr int TK = xtile(L);
+    if (g.TKx != TK) throw std::runtime_error("basic x tile mismatch");
+    const size_t sm = ((size_t)L * TK * 12 + L) * sizeof(cx<double>);
+    if (sm > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kxb<L, TK>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sm);
+    hipLaunchKernelGGL((kxb<L, TK>), dim3(g.nbx), dim3(XNT), sm, st, g, p);
+    return true;
+}
+
+} // namespace
+
+void launch_z(int beh, int mode, const Geo& g, const BPtr<double>& p, hipStream_t st)
+{
+    if (beh == BEH_EP) {
+        if (mode == BMODE_START) z_mode<BEH_EP, BMODE_START>(g, p, st);
+        else z_mode<BEH_EP, BMODE_ITER>(g, p, st);
+    } else {
+        if (mode == BMODE_START) z_mode<BEH_ED, BMODE_START>(g, p, st);
+        else z_mode<BEH_ED, BMODE_ITER>(g, p, st);
+    }
+}
+
+void launch_y12(const Geo& g, const Ptr<double>& p, hipStream_t st) { y_run<Comp12, -1, BUF_K>(g, p, st); }
+
+void launch_y6(const Geo& g, const Ptr<double>& p, hipStream_t st) { y_run<Comp6, 1, BUF_MAIN>(g, p, st); }
+
+void launch_x(const Geo& g, const BPtr<double>& p, hipStream_t st)
+{
+    const bool ok = x_at<256>(g, p, st) || x_at<192>(g, p, st) || x_at<128>(g, p, st) || x_at<96>(g, p, st) ||
+                    x_at<64>(g, p, st) || x_at<48>(g, p, st) || x_at<32>(g, p, st);
+    if (!ok) throw std::runtime_error("no basic x kernel for this grid");
+}
+
+} // namespace bfast
+} // namespace m3d

@@ -485,6 +485,7 @@ __global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const doubl
 }
 
 // after the x pass: eqVar of this iterate, diff_avr_eqVar, stop rule (:820-826)
+template <int U = 0> // a template so that every unit including this header may instantiate it
 __global__ void __launch_bounds__(FTHREADS) k_bfin2(Geo g, Ctl* ctl, const double* pchk)
 {
     __shared__ double red[FTHREADS / 64];

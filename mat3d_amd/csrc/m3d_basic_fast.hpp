@@ -1,0 +1,52 @@
+// m3d_basic_fast.hpp -- interface of the compile-time specialised kernels of
+// the basic scheme ("ep" / "ed", FFT_analysis3D.cpp:796-853), defined in
+// m3d_basic.hip.  Same passes as the runtime-planned kernels of
+// m3d_basic.hpp (z pass with the constitutive update, y pass on 12
+// components, x pass with eqVar partials and the Green step, y pass on 6),
+// with every FFT length and tile a template constant.
+#pragma once
+#include "m3d_basic.hpp"
+#include "m3d_fast.hpp"
+
+namespace m3d {
+namespace bfast {
+
+constexpr int ZNT = 128; // threads of the z pass
+constexpr int XNT = 256; // threads of the x pass
+
+// z tile: TY lines (y) of 12 components each, TY*12*LZ <= 2304 complex
+constexpr int zlines(int LZ, int Ny)
+{
+    int ty = 1;
+    while (ty < 8 && (ty * 2) * 12 * LZ <= 2304 && Ny % (ty * 2) == 0) ty *= 2;
+    return ty;
+}
+// x tile: TK kz values of 12 components
+constexpr int xtile(int L)
+{
+    const int tk = 3072 / (12 * L);
+    return tk < 1 ? 1 : tk;
+}
+// y tiles (the generic fast y pass with 12 / 6 components)
+constexpr int ytile12(int L) { return fast::xytile<double>(L, 12); }
+constexpr int ytile6(int L) { return fast::xytile<double>(L, 6); }
+
+// line lengths with specialised basic kernels: z half-lengths and x/y lengths
+inline bool z_supported(int LZ) { return LZ == 16 || LZ == 24 || LZ == 32 || LZ == 48 || LZ == 64 || LZ == 96 || LZ == 128; }
+inline bool xy_supported(int L) { return L == 32 || L == 48 || L == 64 || L == 96 || L == 128 || L == 192 || L == 256; }
+inline bool supported(int Nx, int Ny, int Nz)
+{
+    return Nz % 2 == 0 && z_supported(Nz / 2) && xy_supported(Nx) && xy_supported(Ny) && Ny % zlines(Nz / 2, Ny) == 0 &&
+           zlines(Nz / 2, Ny) == zlines(Nz / 2, Nz);
+}
+
+// host launchers (m3d_basic.hip); g carries Nx..Nzh, N, H, invN, nbz, nbx
+// of the tiles above, TY, TKx; the y launch takes the 12- or 6-component
+// view of the spectra in p (S: 6-component Green output, K: 12-component)
+void launch_z(int beh, int mode, const Geo& g, const BPtr<double>& p, hipStream_t st);
+void launch_y12(const Geo& g, const Ptr<double>& p, hipStream_t st);
+void launch_x(const Geo& g, const BPtr<double>& p, hipStream_t st);
+void launch_y6(const Geo& g, const Ptr<double>& p, hipStream_t st);
+
+} // namespace bfast
+} // namespace m3d

@@ -1,6 +1,7 @@
 // m3d_basic_host.hpp -- host driver of the basic scheme ("ep" / "ed").
 #pragma once
 #include "m3d_basic.hpp"
+#include "m3d_basic_fast.hpp"
 #include "m3d_engine.hpp"
 
 namespace {
@@ -10,6 +11,8 @@ namespace {
 struct BasicRunner {
     Engine<Elastic>& e;
     Geo g12{}, g6{};
+    Geo gf{};          // tiles of the compile-time specialised kernels
+    bool fast = false; // grid has specialised basic kernels (m3d_basic.hip)
     int beh;
     double* f6 = nullptr;   // eps, sig, sig_old, eps_old (4 x 6N)
     double* f1 = nullptr;   // ep|d, phi, dep, seff (4 x N)
@@ -36,14 +39,26 @@ struct BasicRunner {
         plan_tiles(g12, sizeof(cx<double>));
         plan_tiles(g6, sizeof(cx<double>));
         g12.nsolve = g6.nsolve = 1;
+        fast = bfast::supported(e.Nx, e.Ny, e.Nz);
+        if (fast) {
+            gf = g12;
+            gf.TY = bfast::zlines(e.Nz / 2, e.Ny);
+            gf.nty = e.Ny / gf.TY;
+            gf.nbz = e.Nx * gf.nty;
+            gf.TKx = bfast::xtile(e.Nx);
+            gf.nkx = (gf.Nzh + gf.TKx - 1) / gf.TKx;
+            gf.nbx = e.Ny * gf.nkx;
+        }
+        const int nbz = fast ? std::max(g12.nbz, gf.nbz) : g12.nbz;
+        const int nbx = fast ? std::max(g12.nbx, gf.nbx) : g12.nbx;
         const size_t N = (size_t)e.N, H = (size_t)e.H;
         try {
             e.alloc(&f6, sizeof(double) * 24 * N);
             e.alloc(&f1, sizeof(double) * 4 * N);
             e.alloc(&S6, sizeof(cx<double>) * 12 * H);
             e.alloc(&S12, sizeof(cx<double>) * 12 * H);
-            e.alloc(&psum, sizeof(double) * NBSUM * g12.nbz);
-            e.alloc(&pchk, sizeof(double) * (g12.nbx + 1));
+            e.alloc(&psum, sizeof(double) * NBSUM * nbz);
+            e.alloc(&pchk, sizeof(double) * (nbx + 1));
             e.alloc(&bs, sizeof(double) * 14 * nblk);
             e.alloc(&mat, sizeof(double) * 256 * 8);
             e.alloc(&ctl, sizeof(Ctl));
@@ -130,13 +145,29 @@ struct BasicRunner {
         }
         hipLaunchKernelGGL((k_ypass<double, 12, -1, BUF_K, GEN>), dim3(g12.Nx * g12.nky), dim3(XYTHREADS), ysm12, st, g12, yp);
         hipLaunchKernelGGL((k_xbasic<double, GEN>), dim3(g12.nbx), dim3(XYTHREADS), xsm, st, g12, bp);
-        hipLaunchKernelGGL(k_bfin2, dim3(1), dim3(FTHREADS), 0, st, g12, ctl, pchk);
+        hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, g12, ctl, pchk);
         hipLaunchKernelGGL((k_ypass<double, 6, 1, BUF_MAIN, GEN>), dim3(g6.Nx * g6.nky), dim3(XYTHREADS), ysm6, st, g6, yp);
+    }
+
+    // one stage with the specialised kernels (same sequence as stage())
+    void stage_fast(int mode)
+    {
+        hipStream_t st = e.st;
+        BPtr<double> bp = bptr();
+        Ptr<double> yp = yptr();
+        bfast::launch_z(beh, mode, gf, bp, st);
+        if (mode == BMODE_START) hipLaunchKernelGGL((k_bfin1<BMODE_START>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
+        else hipLaunchKernelGGL((k_bfin1<BMODE_ITER>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
+        bfast::launch_y12(gf, yp, st);
+        bfast::launch_x(gf, bp, st);
+        hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, pchk);
+        bfast::launch_y6(gf, yp, st);
     }
 
     void run_stage(int mode)
     {
-        if (g12.gen) stage<true>(mode);
+        if (fast) stage_fast(mode);
+        else if (g12.gen) stage<true>(mode);
         else stage<false>(mode);
     }
 
@@ -175,6 +206,26 @@ struct BasicRunner {
         int nx = 0, total_it = 0;
         std::vector<double> part((size_t)14 * nblk);
         const int ck = 4;
+        // one captured graph of `ck` iterations, replayed in every increment
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < ck; ++k) run_stage(BMODE_ITER);
+        HCHK(hipStreamEndCapture(st, &graph));
+        hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ge != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            throw HipError{ge, "hipGraphInstantiate"};
+        }
+        struct GraphGuard {
+            hipGraph_t g;
+            hipGraphExec_t x;
+            ~GraphGuard()
+            {
+                (void)hipGraphExecDestroy(x);
+                (void)hipGraphDestroy(g);
+            }
+        } guard_graph{graph, exec};
         HCHK(hipEventRecord(e.ev0, st));
         for (int inc = 0; inc < nInc; ++inc) {
             // host copy of the device state, then reset the per-increment fields
@@ -187,29 +238,12 @@ struct BasicRunner {
             HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
             run_stage(BMODE_START);
             HCHK(hipGetLastError());
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < ck; ++k) run_stage(BMODE_ITER);
-            HCHK(hipStreamEndCapture(st, &graph));
-            hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (ge != hipSuccess) {
-                (void)hipGraphDestroy(graph);
-                throw HipError{ge, "hipGraphInstantiate"};
-            }
             for (int guard = 0; guard < 100000; ++guard) {
-                hipError_t le = hipGraphLaunch(exec, st);
-                if (le == hipSuccess) le = hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st);
-                if (le == hipSuccess) le = hipStreamSynchronize(st);
-                if (le != hipSuccess) {
-                    (void)hipGraphExecDestroy(exec);
-                    (void)hipGraphDestroy(graph);
-                    throw HipError{le, "basic scheme graph"};
-                }
+                HCHK(hipGraphLaunch(exec, st));
+                HCHK(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
+                HCHK(hipStreamSynchronize(st));
                 if (c.done) break;
             }
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
             if (c.done == 3)
                 throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};
             total_it += c.it;

@@ -65,6 +65,9 @@ int guarded(Hd* h, F&& f)
     } catch (const std::bad_alloc&) {
         h->err = "host allocation failed";
         return M3D_ERR_OOM;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return M3D_ERR_UNSUPPORTED;
     } catch (...) {
         h->err = "unexpected error";
         return M3D_ERR_HIP;

@@ -167,8 +167,12 @@ __device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w)
     return cmul(a, w);
 }
 
-// one Stockham pass over W interleaved lines of length L (in place)
-template <typename T, int L, int W, int NT, int R, int Ns, int DIR, int TWS>
+// one Stockham pass over W interleaved lines of length L (in place).  The
+// LDS row of sample j holds WS >= W columns; line l sits in column
+// (l / G) * GS + OFF + l % G (defaults: columns 0..W-1), so a subset of the
+// lines (e.g. 6 of every 12) can be transformed in place.
+template <typename T, int L, int W, int NT, int R, int Ns, int DIR, int TWS, int WS = W, int G = W, int GS = 0,
+          int OFF = 0>
 __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
 {
     constexpr int nR = L / R;
@@ -181,10 +185,11 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
     for (int i = 0; i < IPT; ++i) {
         const int it = tid + i * NT;
         if (items % NT == 0 || it < items) {
-            const int l = it % W, j = it / W, k = j % Ns;
+            const int l0 = it % W, j = it / W, k = j % Ns;
+            const int l = (l0 / G) * GS + OFF + l0 % G;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cx<T> a = s[(j + r * nR) * W + l];
+                cx<T> a = s[(j + r * nR) * WS + l];
                 if (Ns > 1 && r > 0) a = twmul<DIR>(a, tw[k * r * tstep]);
                 v[i][r] = a;
             }
@@ -196,23 +201,32 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
     for (int i = 0; i < IPT; ++i) {
         const int it = tid + i * NT;
         if (items % NT == 0 || it < items) {
-            const int l = it % W, j = it / W, k = j % Ns;
+            const int l0 = it % W, j = it / W, k = j % Ns;
+            const int l = (l0 / G) * GS + OFF + l0 % G;
             const int ob = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[(ob + r * Ns) * W + l] = v[i][r];
+            for (int r = 0; r < R; ++r) s[(ob + r * Ns) * WS + l] = v[i][r];
         }
     }
     __syncthreads();
 }
 
-template <typename T, int L, int W, int NT, int DIR, int TWS, int P = 0, int Ns = 1>
+template <typename T, int L, int W, int NT, int DIR, int TWS, int P = 0, int Ns = 1, int WS = W, int G = W, int GS = 0,
+          int OFF = 0>
 __device__ __forceinline__ void fft(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
 {
     if constexpr (P < npass(L)) {
         constexpr int R = radix(L, P);
-        pass<T, L, W, NT, R, Ns, DIR, TWS>(s, tw);
-        fft<T, L, W, NT, DIR, TWS, P + 1, Ns * R>(s, tw);
+        pass<T, L, W, NT, R, Ns, DIR, TWS, WS, G, GS, OFF>(s, tw);
+        fft<T, L, W, NT, DIR, TWS, P + 1, Ns * R, WS, G, GS, OFF>(s, tw);
     }
+}
+
+// the lines OFF + (0..G-1) of every group of GS columns (WS columns in all)
+template <typename T, int L, int W, int NT, int DIR, int TWS, int WS, int G, int GS, int OFF>
+__device__ __forceinline__ void fft_sub(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
+{
+    fft<T, L, W, NT, DIR, TWS, 0, 1, WS, G, GS, OFF>(s, tw);
 }
 
 // vector load/store of n contiguous values into a register array (16-byte
@@ -622,7 +636,7 @@ __global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo 
     __syncthreads();
     fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1>(sl, tw);
 
-    if (KIND == 1) {
+    if constexpr (KIND == 1) {
         const T* prm = prm_all + 2 * p.rve_of[s];
         const int y = o;
         for (int f = tid; f < L * nk; f += NT) {
@@ -639,7 +653,7 @@ __global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo 
         __syncthreads();
         fft<T, L, W, NT, 1, 1>(sl, tw);
     }
-    if (KIND == 2) {
+    if constexpr (KIND == 2) {
         const int y = o;
         double acc = 0.0;
         for (int f = tid; f < L * nk; f += NT) {

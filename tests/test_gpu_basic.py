@@ -50,6 +50,26 @@ def test_j2_matches_oracle(shape):
     assert rel(s, s_o) < 1e-9 and rel(e, e_o) < 1e-9
 
 
+@pytest.mark.parametrize("shape,behav", [((32, 32, 32), "ep"), ((32, 64, 48), "ep"), ((32, 32, 32), "ed")])
+def test_specialised_kernels_match_oracle(shape, behav):
+    """Grids whose line lengths have the compile-time specialised basic-scheme
+    kernels (the tests above run the runtime-planned ones)."""
+    n = max(shape)
+    ids = gen.spheres(n, 4, 0.25, seed=4)[: shape[0], : shape[1], : shape[2]].copy()
+    X0 = np.eye(6)[0] if behav == "ep" else np.eye(6)[1]
+    mat = gen.MAT_C4 if behav == "ep" else MAT_ED
+    E = 0.02 * X0 if behav == "ep" else 0.01 * X0
+    (P_o, e_o, s_o, it_o), (P, e, s, info) = run_both(shape, behav, mat, ids, E, X0, 4)
+    print(f"[basic fast {shape} {behav}] iters oracle {it_o} gpu {info['iters_fp64']} last row {P[-1, :3]}")
+    assert info["iters_fp64"] == it_o
+    assert P.shape == P_o.shape
+    assert rel(P, P_o) < 1e-9
+    # past the damage softening (thousands of iterations) FFT rounding
+    # differences grow to ~1e-9 in the fields
+    ftol = 1e-9 if behav == "ep" else 1e-7
+    assert rel(s, s_o) < ftol and rel(e, e_o) < ftol
+
+
 def test_damage_matches_oracle():
     n = 16
     ids = gen.spheres(n, 3, 0.25, seed=6)
