@@ -89,6 +89,12 @@ int m3d_elastic_run(m3d_elastic *h, double tol_field, double tol_eq, int nInc, m
  * or d (ed), mean eps_p (ep) or 0].  prop may be NULL to query *rows. */
 int m3d_elastic_get_prop(m3d_elastic *h, double *prop, int *rows);
 
+/* public members epsilon_p_eff and sigma_eff ("ep"; sig_eff is zero for
+ * "ed") or d ("ed") after runAnalysis, N values each (either pointer may be
+ * NULL) -- FFT_analysis3D.hpp:56-98, read by printer / vtiWriter
+ * (src/3d/FFT_analysis3D.cpp:1001-1104) */
+int m3d_elastic_get_state(m3d_elastic *h, double *ep_eff_or_d, double *sig_eff);
+
 /* double homogenization(double tol_field, double tol_eq, string out)
  * -- include/FFT_analysis3D.hpp:209, src/3d/FFT_analysis3D.cpp:856-999.
  * The six load cases run as one batched launch.  Outputs (column-major
@@ -112,7 +118,11 @@ int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0,
  * (default 1e-6), "rollback" (FP32 iterations redone in FP64 when the switch
  * iterate already satisfies the stop rule within 2 x tol, default 5), "cap"
  * (iteration cap, default 2000 elastic / 500 thermal), "chunk" (iterations
- * per graph replay, 0 = automatic). */
+ * per graph replay, 0 = automatic), "refine" (1, default: the polish iterates
+ * an FP32 correction of one FP64 anchor with FP64 stop-rule sums and checks;
+ * 0: plain FP64 polish iterations), "scheme" (0, default: the reference's
+ * accelerated "e" loop; 1: the basic Moulinec-Suquet iteration with the load
+ * held fixed, for homogenization and the "e" runAnalysis). */
 int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 
 /* Kernel timing for measurement tools (no reference counterpart).  Times

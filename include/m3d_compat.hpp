@@ -14,24 +14,21 @@
 // e.g. src/3d/FFT_analysis3D.cpp:672-673).
 #pragma once
 
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "m3d.h"
+#include "m3d_io.hpp"
 
 namespace m3d {
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
 
 class FFT_Analysis3D {
   public:
     // FFT_Analysis3D(int Nx1, int Ny1, int Nz1, string mater_behav), FFT_analysis3D.hpp:196
     FFT_Analysis3D(int Nx1, int Ny1, int Nz1, const std::string& mater_behav, int precision = M3D_FP32_POLISH)
-        : Nx(Nx1), Ny(Ny1), Nz(Nz1), xyz(Nx1 * Ny1 * Nz1)
+        : Nx(Nx1), Ny(Ny1), Nz(Nz1), xyz(Nx1 * Ny1 * Nz1), behav_(mater_behav)
     {
         int rc = m3d_elastic_create(Nx1, Ny1, Nz1, mater_behav.c_str(), precision, &h_);
         if (rc != M3D_OK) throw Error(rc, "m3d_elastic_create failed");
@@ -50,6 +47,8 @@ class FFT_Analysis3D {
         for (int j = 0; j < 7; ++j)
             for (int i = 0; i < noPhases; ++i) m[i + (size_t)j * noPhases] = mat.data()[i + (size_t)j * mat.rows()];
         chk(m3d_elastic_material(h_, m.data(), ids.data(), noPhases));
+        El_.assign(m.begin(), m.begin() + noPhases);
+        v_.assign(m.begin() + noPhases, m.begin() + 2 * noPhases);
         vol_fractions.assign(noPhases, 0.0);
         int sb = 1;
         chk(m3d_elastic_info(h_, vol_fractions.data(), &sb, &la0, &mu0));
@@ -69,10 +68,24 @@ class FFT_Analysis3D {
     {
         chk(m3d_elastic_run(h_, tol_field, tol_eq, nInc1, &info[0]));
         fetch_fields(0);
+        last_run_basic_ = behav_ != "e";
+        if (last_run_basic_) {
+            // prop_homogenized (nInc x 9) and the per-voxel history
+            int rows = 0;
+            chk(m3d_elastic_get_prop(h_, nullptr, &rows));
+            nInc = rows;
+            prop_homogenized.assign((size_t)9 * rows, 0.0);
+            chk(m3d_elastic_get_prop(h_, prop_homogenized.data(), &rows));
+            std::vector<double> a((size_t)xyz), b((size_t)xyz);
+            chk(m3d_elastic_get_state(h_, a.data(), b.data()));
+            if (behav_ == "ep") { epsilon_p_eff = a; sigma_eff = b; }
+            else d = a;
+        }
     }
 
-    // double homogenization(double, double, string), FFT_analysis3D.hpp:209
-    double homogenization(double tol_field, double tol_eq, const std::string& /*outputName*/ = "")
+    // double homogenization(double, double, string), FFT_analysis3D.hpp:209;
+    // a non-empty outputName writes the report outputName + ".txt" (:859)
+    double homogenization(double tol_field, double tol_eq, const std::string& outputName = "")
     {
         CHomo.resize(36); SHomo.resize(36); epHomo.resize(36); sigHomo.resize(36);
         double ret = 0;
@@ -80,7 +93,34 @@ class FFT_Analysis3D {
                                        sigHomo.data(), &ret, info));
         E11 = 1. / SHomo[0]; E22 = 1. / SHomo[7]; E33 = 1. / SHomo[14];
         fetch_fields(5); // the reference keeps the fields of the last load case
+        last_run_basic_ = false;
+        if (!outputName.empty()) {
+            std::ofstream homo(outputName + ".txt");
+            io::ElasticReport r{Nx, Ny, Nz, strainBase, El_, v_, vol_fractions,
+                                CHomo.data(), SHomo.data(), epHomo.data(), sigHomo.data()};
+            io::write_report(homo, r);
+        }
         return ret;
+    }
+
+    // void printer(string outFldr), FFT_analysis3D.hpp:211
+    void printer(const std::string& outFldr = "results")
+    {
+        const double* h = behav_ == "ep" ? epsilon_p_eff.data() : (behav_ == "ed" ? d.data() : nullptr);
+        io::printer(outFldr, last_run_basic_ ? behav_ : std::string("e"), xyz, epsilon.data(), sigma.data(),
+                    last_run_basic_ ? h : nullptr, last_run_basic_ ? prop_homogenized.data() : nullptr, nInc);
+    }
+
+    // void vtiWriter(string fileName), FFT_analysis3D.hpp:213
+    void vtiWriter(const std::string& fileName)
+    {
+        const double* h = nullptr;
+        const double* se = nullptr;
+        if (last_run_basic_) {
+            h = behav_ == "ep" ? epsilon_p_eff.data() : d.data();
+            se = behav_ == "ep" ? sigma_eff.data() : nullptr;
+        }
+        io::write_vti(fileName, Nx, Ny, Nz, 14, {{epsilon.data(), 6}, {sigma.data(), 6}, {h, 1}, {se, 1}});
     }
 
     void set_option(const char* key, double v) { chk(m3d_elastic_set_option(h_, key, v)); }
@@ -92,6 +132,9 @@ class FFT_Analysis3D {
     bool strainBase = true;
     std::vector<double> vol_fractions, CHomo, SHomo, epHomo, sigHomo;
     std::vector<double> epsilon, sigma; // 6 x N column-major
+    std::vector<double> prop_homogenized; // nInc x 9 column-major ("ep"/"ed")
+    std::vector<double> epsilon_p_eff, sigma_eff, d; // N ("ep": the first two, "ed": d)
+    int nInc = 0;
     m3d_run_info info[6] = {};
 
   private:
@@ -99,6 +142,9 @@ class FFT_Analysis3D {
     {
         if (rc != M3D_OK) throw Error(rc, m3d_elastic_last_error(h_));
     }
+    std::string behav_;
+    bool last_run_basic_ = false;
+    std::vector<double> El_, v_;
     void fetch_fields(int lc)
     {
         epsilon.resize((size_t)6 * xyz);
@@ -128,6 +174,10 @@ class FFT_analysis3d_thermal {
         std::vector<int32_t> ids(RVE1.data(), RVE1.data() + xyz);
         chk(m3d_thermal_material(h_, km, ki, ids.data()));
         if (have_load_) chk(m3d_thermal_set_load(h_, E));
+        km_ = km;
+        ki_ = ki;
+        rve_sum_ = 0;
+        for (int32_t q : ids) rve_sum_ += q;
     }
 
     // void setLoad(MatrixDdd& E1), FFT_thermal3D.hpp:140.  The reference example
@@ -149,13 +199,29 @@ class FFT_analysis3d_thermal {
     }
 
     // void homogenization(double, double, string), FFT_thermal3D.hpp:148
-    void homogenization(double tol_field, double tol_eq, const std::string& /*filename*/ = "")
+    // a non-empty filename writes the report filename + ".txt" (:491)
+    void homogenization(double tol_field, double tol_eq, const std::string& filename = "")
     {
         KHomo.resize(9); RHomo.resize(9); tempGradHomo.resize(9); heatFluxHomo.resize(9);
         chk(m3d_thermal_homogenization(h_, tol_field, tol_eq, KHomo.data(), RHomo.data(), tempGradHomo.data(),
                                        heatFluxHomo.data(), info));
         K11 = KHomo[0]; K22 = KHomo[4]; K33 = KHomo[8];
         fetch_fields(2);
+        if (!filename.empty()) {
+            std::ofstream homo(filename + ".txt");
+            io::ThermalReport r{Nx, Ny, Nz, km_, ki_, rve_sum_,
+                                KHomo.data(), RHomo.data(), tempGradHomo.data(), heatFluxHomo.data()};
+            io::write_report(homo, r);
+        }
+    }
+
+    // void printer(string folderName), FFT_thermal3D.hpp:150
+    void printer(const std::string& folderName) { io::printer_thermal(folderName, xyz, tempGradient.data(), heatFlux.data()); }
+
+    // void vtiWriter(string fileName), FFT_thermal3D.hpp:152
+    void vtiWriter(const std::string& fileName)
+    {
+        io::write_vti(fileName, Nx, Ny, Nz, 6, {{tempGradient.data(), 3}, {heatFlux.data(), 3}});
     }
 
     void set_option(const char* key, double v) { chk(m3d_thermal_set_option(h_, key, v)); }
@@ -180,6 +246,8 @@ class FFT_analysis3d_thermal {
     }
     m3d_thermal* h_ = nullptr;
     bool have_load_ = false;
+    double km_ = 0, ki_ = 0;
+    long long rve_sum_ = 0;
 };
 
 } // namespace m3d

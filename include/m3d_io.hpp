@@ -1,0 +1,433 @@
+// m3d_io.hpp -- header-only on-disk formats around the solver (used by
+// m3d_compat.hpp, usable on their own): the example drivers' input decks,
+// the homogenization reports, the printer text dumps and a dependency-free
+// VTK XML ImageData (.vti) writer.
+//
+//   read_struct_deck / read_therm_deck  example/3d/main.cpp:8-68,
+//                                       example/3d_thermal/main.cpp:8-56
+//   read_rve                            example/3d/main.cpp:72-93
+//   write_report (elastic)              src/3d/FFT_analysis3D.cpp:905-996
+//   write_report (thermal)              src/3dTherm/FFT_thermal3D.cpp:581-620
+//   printer                             src/3d/FFT_analysis3D.cpp:1001-1053,
+//                                       src/3dTherm/FFT_thermal3D.cpp:626-691
+//   write_vti                           src/3d/FFT_analysis3D.cpp:1056-1104,
+//                                       src/3dTherm/FFT_thermal3D.cpp:696-740
+//
+// Text output goes through iostreams with the stream's default formatting,
+// and matrices are printed the way Eigen's default stream operator prints
+// them (columns right-aligned to the widest entry, one space apart), so a
+// report reads as the reference's does.  Deviations: the .vti file is
+// written by this header in ASCII (no VTK library), with the stress in
+// components 6..11 (the reference reads sigma(i+6, .), past the matrix).
+#pragma once
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include <stdexcept>
+
+#include "m3d.h"
+
+namespace m3d {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+namespace io {
+
+// ------------------------------------------------------------------ decks
+// The example drivers read "key value" pairs; a value whose atof() is
+// nonzero is a number, anything else a string, in order of appearance; the
+// last value is the load vector "a,b,c,...".
+struct Deck {
+    std::vector<double> num;
+    std::vector<std::string> str;
+    std::vector<double> load;
+};
+
+inline Deck read_deck(const std::string& path, size_t max_num, size_t nload)
+{
+    std::ifstream in(path);
+    if (!in) throw Error(M3D_ERR_ARG, "deck: cannot open " + path);
+    Deck d;
+    std::string key, val;
+    while (in >> key) {
+        if (!(in >> val)) break;
+        if (std::atof(val.c_str()) != 0.0 && d.num.size() < max_num) d.num.push_back(std::atof(val.c_str()));
+        else d.str.push_back(val);
+    }
+    std::string rest = val;
+    for (size_t i = 0; i < nload; ++i) {
+        d.load.push_back(std::atof(rest.c_str()));
+        const size_t c = rest.find(',');
+        if (c != std::string::npos) rest = rest.substr(c + 1); // no comma: the value repeats, as there
+    }
+    return d;
+}
+
+// in_struct.txt: num = {N, E_0, v_0, E_1, v_1, tol_stress, tol_equilibrium},
+// str = {rve file, Y|N, vti name, S|H, report name, load}
+struct StructDeck {
+    int N = 0;
+    double E0 = 0, v0 = 0, E1 = 0, v1 = 0, tol_field = 0, tol_eq = 0;
+    std::string rve, vti, report;
+    bool write_vti = false, homogenize = false;
+    double E[6] = {0, 0, 0, 0, 0, 0};
+};
+
+inline StructDeck read_struct_deck(const std::string& path)
+{
+    const Deck d = read_deck(path, 7, 6);
+    if (d.num.size() < 7 || d.str.size() < 5) throw Error(M3D_ERR_ARG, "deck: the input data is incomplete or wrong");
+    if (d.str[1] != "Y" && d.str[1] != "N") throw Error(M3D_ERR_ARG, "deck: vtiOutput_YorN must be Y or N");
+    if (d.str[3] != "S" && d.str[3] != "H") throw Error(M3D_ERR_ARG, "deck: SingleAnalysisOrHomogenization must be S or H");
+    StructDeck s;
+    s.N = (int)d.num[0];
+    s.E0 = d.num[1]; s.v0 = d.num[2]; s.E1 = d.num[3]; s.v1 = d.num[4];
+    s.tol_field = d.num[5]; s.tol_eq = d.num[6];
+    s.rve = d.str[0]; s.write_vti = d.str[1] == "Y"; s.vti = d.str[2];
+    s.homogenize = d.str[3] == "H"; s.report = d.str[4];
+    for (int i = 0; i < 6; ++i) s.E[i] = d.load[i];
+    return s;
+}
+
+// in_therm.txt: num = {N, k_1 (matrix), k_0 (inclusion), tol_flux, tol_equilibrium},
+// str = {rve file, Y|N, vti name, S|H, report name, gradient}
+struct ThermDeck {
+    int N = 0;
+    double km = 0, ki = 0, tol_field = 0, tol_eq = 0;
+    std::string rve, vti, report;
+    bool write_vti = false, homogenize = false;
+};
+
+inline ThermDeck read_therm_deck(const std::string& path)
+{
+    const Deck d = read_deck(path, 5, 3);
+    if (d.num.size() < 5 || d.str.size() < 5) throw Error(M3D_ERR_ARG, "deck: the input data is incomplete or wrong");
+    if (d.str[1] != "Y" && d.str[1] != "N") throw Error(M3D_ERR_ARG, "deck: vtiOutput_YorN must be Y or N");
+    if (d.str[3] != "S" && d.str[3] != "H") throw Error(M3D_ERR_ARG, "deck: SingleAnalysisOrHomogenization must be S or H");
+    ThermDeck t;
+    t.N = (int)d.num[0];
+    t.km = d.num[1]; t.ki = d.num[2]; t.tol_field = d.num[3]; t.tol_eq = d.num[4];
+    t.rve = d.str[0]; t.write_vti = d.str[1] == "Y"; t.vti = d.str[2];
+    t.homogenize = d.str[3] == "H"; t.report = d.str[4];
+    return t;
+}
+
+// one phase id per voxel, whitespace separated, voxel order z fastest
+inline std::vector<int32_t> read_rve(const std::string& path, long long N)
+{
+    std::ifstream in(path);
+    if (!in) throw Error(M3D_ERR_ARG, "rve: cannot open " + path);
+    std::vector<int32_t> ids;
+    ids.reserve((size_t)N);
+    double a;
+    while (in >> a) ids.push_back((int32_t)a);
+    if ((long long)ids.size() != N) throw Error(M3D_ERR_ARG, "rve: dimension is not correct");
+    return ids;
+}
+
+// ------------------------------------------------------------- formatting
+// A column-major rows x cols matrix the way Eigen's operator<< prints it
+// with the default format: each entry in the stream's current format,
+// right-aligned to the widest entry, entries one space apart, rows on
+// separate lines, no trailing newline.
+inline void print_matrix(std::ostream& os, const double* a, int rows, int cols)
+{
+    std::streamsize width = 0;
+    for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i) {
+            std::stringstream ss;
+            ss.copyfmt(os);
+            ss << a[i + (size_t)j * rows];
+            width = std::max<std::streamsize>(width, (std::streamsize)ss.str().size());
+        }
+    for (int i = 0; i < rows; ++i) {
+        for (int j = 0; j < cols; ++j) {
+            if (j) os << ' ';
+            os.width(width);
+            os << a[i + (size_t)j * rows];
+        }
+        if (i < rows - 1) os << '\n';
+    }
+}
+
+inline std::vector<double> inverse(std::vector<double> A, int n)
+{
+    std::vector<double> X((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) X[i + (size_t)i * n] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[r + (size_t)c * n]) > std::fabs(A[p + (size_t)c * n])) p = r;
+        for (int k = 0; k < n; ++k) {
+            std::swap(A[c + (size_t)k * n], A[p + (size_t)k * n]);
+            std::swap(X[c + (size_t)k * n], X[p + (size_t)k * n]);
+        }
+        const double d = A[c + (size_t)c * n];
+        for (int k = 0; k < n; ++k) { A[c + (size_t)k * n] /= d; X[c + (size_t)k * n] /= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[r + (size_t)c * n];
+            for (int k = 0; k < n; ++k) {
+                A[r + (size_t)k * n] -= f * A[c + (size_t)k * n];
+                X[r + (size_t)k * n] -= f * X[c + (size_t)k * n];
+            }
+        }
+    }
+    return X;
+}
+
+// ---------------------------------------------------------------- reports
+// Inputs of an elastic report: grid, base, per-phase E and nu (material()
+// columns 0, 1), volume fractions and the four 6x6 results (column-major).
+struct ElasticReport {
+    int Nx, Ny, Nz;
+    bool strainBase;
+    std::vector<double> El, v, vol;
+    const double *CHomo, *SHomo, *epHomo, *sigHomo;
+};
+
+// homogenization report, FFT_analysis3D.cpp:905-996 (Hashin-Shtrikman
+// bounds of the n-phase composite, Hashin & Shtrikman, JMPS 1963, as there)
+inline void write_report(std::ostream& homo, const ElasticReport& ph)
+{
+    const ElasticReport& a = ph;
+    const int n = (int)ph.El.size();
+    std::vector<double> mu(n), la(n), K(n);
+    for (int i = 0; i < n; ++i) {
+        mu[i] = ph.El[i] / (1 + ph.v[i]) / 2.;
+        la[i] = ph.v[i] * ph.El[i] / (1 + ph.v[i]) / (1 - 2 * ph.v[i]);
+        K[i] = la[i] + (2. / 3.) * mu[i];
+    }
+    const double Gn = *std::max_element(mu.begin(), mu.end()), G1 = *std::min_element(mu.begin(), mu.end());
+    const double Kn = *std::max_element(K.begin(), K.end()), K1 = *std::min_element(K.begin(), K.end());
+    const double alfa1 = -3 / (3 * K1 + 4 * G1), alfan = -3 / (3 * Kn + 4 * Gn);
+    const double beta1 = -(3 * (K1 + 2 * G1)) / ((3 * K1 + 4 * G1) * 5 * G1);
+    const double betan = -(3 * (Kn + 2 * Gn)) / ((3 * Kn + 4 * Gn) * 5 * Gn);
+    double A1 = 0, An = 0, B1 = 0, Bn = 0;
+    for (int i = 0; i < n; ++i) {
+        if (K[i] != K1) A1 += ph.vol[i] / ((1. / (K[i] - K1)) - alfa1);
+        if (K[i] != Kn) An += ph.vol[i] / ((1. / (K[i] - Kn)) - alfan);
+        if (mu[i] != G1) B1 += ph.vol[i] / ((0.5 / (mu[i] - G1)) - beta1);
+        if (mu[i] != Gn) Bn += ph.vol[i] / ((0.5 / (mu[i] - Gn)) - betan);
+    }
+    const double K_min = K1 + A1 / (1 + alfa1 * A1), K_max = Kn + An / (1 + alfan * An);
+    const double G_min = G1 + 0.5 * B1 / (1 + beta1 * B1), G_max = Gn + 0.5 * Bn / (1 + betan * Bn);
+    const double E_min = 9 * K_min * G_min / (3 * K_min + G_min), E_max = 9 * K_max * G_max / (3 * K_max + G_max);
+    const char* rule = "      ----------------------------------------------------------------";
+
+    homo << "*************************************************************************" << std::endl << std::endl;
+    homo << "        F F T     H O M O G E N I Z A T I O N    B Y   M A T 3 D" << std::endl << std::endl;
+    if (a.strainBase) homo << "                      Stran-Based Algorithm Is Used" << std::endl;
+    else homo << "                      Stress-Based Algorithm Is Used" << std::endl;
+    homo << "                           Nx=" << a.Nx << "   " << "Ny=" << a.Ny << "   " << "Nz=" << a.Nz << std::endl
+         << std::endl;
+    homo << rule << std::endl;
+    for (int i = 0; i < n; ++i) {
+        const std::string k = std::to_string(i) + " = ";
+        homo << "                      Volume Fraction Phase " << k << ph.vol[i] << std::endl;
+        homo << "     E" << k << ph.El[i] << "     " << "v" << k << ph.v[i] << "     " << "K" << k << K[i] << "     "
+             << "G" << k << mu[i] << "     " << std::endl << std::endl;
+    }
+    homo << rule << std::endl << std::endl;
+    homo << "                        Hashin-Shtrikman Bounds" << std::endl;
+    homo << "         E_min=" << E_min << "   " << "K_min=" << K_min << "   " << "G_min=" << G_min << std::endl;
+    homo << "         E_max=" << E_max << "   " << "K_max=" << K_max << "   " << "G_max=" << G_max << std::endl
+         << std::endl << std::endl;
+    homo << "************************************************************************" << std::endl << std::endl;
+    homo << "Homogenized Elastic Modules:" << std::endl;
+    homo << "E11=" << 1 / a.SHomo[0] << "     " << "E22=" << 1 / a.SHomo[7] << "     " << "E33=" << 1 / a.SHomo[14]
+         << "     " << std::endl << std::endl;
+    homo << std::endl << "C_Homogenized=" << std::endl << std::endl;
+    print_matrix(homo, a.CHomo, 6, 6);
+    homo << std::endl << rule << std::endl;
+    homo << std::endl << "S_Homogenized=" << std::endl << std::endl;
+    print_matrix(homo, a.SHomo, 6, 6);
+    homo << std::endl << rule << std::endl;
+    for (int i = 0; i < n; ++i) {
+        std::vector<double> C(36, 0.0);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) C[r + 6 * c] = la[i] + (r == c ? 2 * mu[i] : 0.0);
+        for (int r = 3; r < 6; ++r) C[r + 6 * r] = 2 * mu[i];
+        const std::vector<double> S = inverse(C, 6);
+        homo << std::endl << "C" << std::to_string(i) << "=" << std::endl;
+        print_matrix(homo, C.data(), 6, 6);
+        homo << std::endl << rule << std::endl;
+        homo << std::endl << "S" << std::to_string(i) << "=" << std::endl;
+        print_matrix(homo, S.data(), 6, 6);
+        homo << std::endl << rule << std::endl;
+    }
+    homo << std::endl
+         << "Homogenized Load Cases (Each row is one load case i.e (E11 E22 E33 E32 E31 E12) and Eij is summation of "
+            "strain components in in  all voxels )"
+         << std::endl << std::endl;
+    print_matrix(homo, a.epHomo, 6, 6);
+    homo << std::endl;
+    homo << std::endl
+         << "Homogenized stess (Each coulemn is (SS11 SS22 SS33 SS32 SS31 SS12) and SSij is summation of stress "
+            "components in in all voxels )"
+         << std::endl << std::endl;
+    print_matrix(homo, a.sigHomo, 6, 6);
+    homo << std::endl;
+}
+
+// thermal homogenization report, FFT_thermal3D.cpp:581-620.  rve_sum: the sum
+// of the phase ids; the white-phase fraction RVE.sum()/xyz is an integer
+// division there (:586) and is kept so.
+struct ThermalReport {
+    int Nx, Ny, Nz;
+    double km, ki;
+    long long rve_sum;
+    const double *KHomo, *RHomo, *tempGradHomo, *heatFluxHomo;
+};
+
+inline void write_report(std::ostream& homo, const ThermalReport& a)
+{
+    const double km = a.km, ki = a.ki;
+    const long long rve_sum = a.rve_sum;
+    const long long xyz = (long long)a.Nx * a.Ny * a.Nz;
+    const double fi_m = (double)(rve_sum / xyz);
+    const double fi_i = 1 - fi_m;
+    const double k_mt = km + (fi_i * (ki - km)) / (1 + fi_m * (ki - km) / (3 * km));
+    const double k_up = fi_i * ki + fi_m * km - fi_i * fi_m * (km - ki) * (km - ki) / (3 * ki - fi_i * (ki - km));
+    const double k_low = fi_i * ki + fi_m * km - fi_i * fi_m * (km - ki) * (km - ki) / (3 * km + fi_m * (ki - km));
+    const double Km[9] = {km, 0, 0, 0, km, 0, 0, 0, km};
+    const double Ki[9] = {ki, 0, 0, 0, ki, 0, 0, 0, ki};
+    const double Rm[9] = {1 / km, 0, 0, 0, 1 / km, 0, 0, 0, 1 / km};
+    homo << "Volume Fraction of the White Phase (or 1)=" << fi_m << std::endl << std::endl;
+    homo << "K phase matrix  ( or 1)=" << std::endl;
+    print_matrix(homo, Km, 3, 3);
+    homo << "     " << std::endl;
+    homo << "K phase inclusion(or 0)=" << std::endl;
+    print_matrix(homo, Ki, 3, 3);
+    homo << "     " << std::endl;
+    homo << "Nx=" << a.Nx << "   " << "Ny=" << a.Ny << "   " << a.Nz << std::endl << std::endl;
+    homo << std::endl << "KHomogenized Mori-Tanaka=" << std::endl << std::endl;
+    homo << k_mt << std::endl;
+    homo << std::endl << "Lower bound HS=" << std::endl << std::endl;
+    homo << k_low << std::endl;
+    homo << std::endl << "Upper bound LS=" << std::endl << std::endl;
+    homo << k_up << std::endl;
+    homo << std::endl << "KHomogenized=" << std::endl << std::endl;
+    print_matrix(homo, a.KHomo, 3, 3);
+    homo << std::endl;
+    homo << std::endl << "RHomogenized(Homogenized Resistivity tensor)=" << std::endl << std::endl;
+    print_matrix(homo, a.RHomo, 3, 3);
+    homo << std::endl;
+    homo << std::endl << "Rm=" << std::endl << std::endl;
+    print_matrix(homo, Rm, 3, 3);
+    homo << std::endl;
+    // the reference prints Km under the "Ri=" heading (:614-615)
+    homo << std::endl << "Ri=" << std::endl << std::endl;
+    print_matrix(homo, Km, 3, 3);
+    homo << std::endl;
+    homo << std::endl << std::endl << "--------------------------------------------------------------------" << std::endl
+         << std::endl;
+    homo << std::endl
+         << "Homogenized Load Cases (Each row is one load case i.e (E11 E22 E33 ) and Eij is summation of temperatur "
+            "gradient components in in  all voxels )"
+         << std::endl << std::endl;
+    print_matrix(homo, a.tempGradHomo, 3, 3);
+    homo << std::endl;
+    homo << std::endl
+         << "Homogenized stess (Each coulemn is (SS11 SS22 SS33) and SSij is summation of heat flux in in all voxels )"
+         << std::endl << std::endl;
+    print_matrix(homo, a.heatFluxHomo, 3, 3);
+    homo << std::endl;
+}
+
+// ---------------------------------------------------------------- printer
+// FFT_analysis3D.cpp:1001-1053: one value per line per component in
+// outFldr/{ep1..ep6,sig1..sig6}.txt and outFldr/ep_p ("ed": outFldr/d, the
+// damage; "e": empty); for "ep"/"ed" also epsilon_sigma.txt (current
+// directory) with the prop_homogenized rows after a zero row.
+inline void printer(const std::string& outFldr, const std::string& behav, long long N, const double* eps,
+                    const double* sig, const double* ep_or_d, const double* prop, int nInc)
+{
+    mkdir(outFldr.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    std::vector<std::string> names = {"ep1.txt",  "ep2.txt",  "ep3.txt",  "ep4.txt",  "ep5.txt",  "ep6.txt", "sig1.txt",
+                                      "sig2.txt", "sig3.txt", "sig4.txt", "sig5.txt", "sig6.txt", "ep_p"};
+    if (behav == "ed") names[12] = "d";
+    for (int j = 0; j < 13; ++j) {
+        std::ofstream out(outFldr + "/" + names[j]);
+        for (long long i = 0; i < N; ++i) {
+            if (j < 6) out << eps[j + 6 * i] << std::endl;
+            else if (j < 12) out << sig[j - 6 + 6 * i] << std::endl;
+            else if (behav != "e" && ep_or_d) out << ep_or_d[i] << std::endl;
+        }
+    }
+    if (behav != "e" && prop) {
+        std::ofstream op("epsilon_sigma.txt");
+        op << "epsilon" << "  " << "sigma1" << "   " << "sigma2" << "   " << "sigma3" << "   " << "tau23" << "   "
+           << "tau13" << "   " << "tau12" << "   " << (behav == "ep" ? "sigma_eff" : "d") << "   " << "epsilon_p_eff"
+           << std::endl;
+        op << 0 << "  " << 0 << "  " << 0 << "  " << 0 << "  " << 0 << "  " << 0 << "  " << 0 << "  " << 0 << "  " << 0
+           << std::endl;
+        for (int i = 0; i < nInc; ++i) {
+            for (int c = 0; c < 9; ++c) op << (c ? "  " : "") << prop[i + (size_t)c * nInc];
+            op << std::endl;
+        }
+    }
+}
+
+// FFT_thermal3D.cpp:626-691: folder/heatFlux{1,2,3}.txt, folder/tempGradient{1,2,3}.txt
+inline void printer_thermal(const std::string& folder, long long N, const double* grad, const double* flux)
+{
+    mkdir(folder.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    for (int c = 0; c < 3; ++c) {
+        std::ofstream q(folder + "/heatFlux" + std::to_string(c + 1) + ".txt");
+        for (long long i = 0; i < N; ++i) q << flux[c + 3 * i] << std::endl;
+    }
+    for (int c = 0; c < 3; ++c) {
+        std::ofstream g(folder + "/tempGradient" + std::to_string(c + 1) + ".txt");
+        for (long long i = 0; i < N; ++i) g << grad[c + 3 * i] << std::endl;
+    }
+}
+
+// --------------------------------------------------------------------- vti
+// VTK XML ImageData with one point-data array "ImageScalars" of ncomp
+// Float64 components; point (x, y, z) of the Nx x Ny x Nz image holds memory
+// index x + Nx (y + Ny z), as the reference's writer fills it.
+inline void write_vti(const std::string& fileName, int Nx, int Ny, int Nz, int ncomp,
+                      const std::vector<std::pair<const double*, int>>& parts)
+{
+    std::ofstream f(fileName);
+    if (!f) throw Error(M3D_ERR_ARG, "vti: cannot open " + fileName);
+    const std::string ext = "0 " + std::to_string(Nx - 1) + " 0 " + std::to_string(Ny - 1) + " 0 " + std::to_string(Nz - 1);
+    f << "<?xml version=\"1.0\"?>\n";
+    f << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    f << "  <ImageData WholeExtent=\"" << ext << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n";
+    f << "    <Piece Extent=\"" << ext << "\">\n";
+    f << "      <PointData Scalars=\"ImageScalars\">\n";
+    f << "        <DataArray type=\"Float64\" Name=\"ImageScalars\" NumberOfComponents=\"" << ncomp
+      << "\" format=\"ascii\">\n";
+    f << std::setprecision(17);
+    const long long N = (long long)Nx * Ny * Nz;
+    for (long long i = 0; i < N; ++i) {
+        int written = 0;
+        for (const auto& pc : parts) {
+            for (int c = 0; c < pc.second; ++c) {
+                f << (written ? " " : "") << (pc.first ? pc.first[(size_t)pc.second * i + c] : 0.0);
+                ++written;
+            }
+        }
+        for (; written < ncomp; ++written) f << " 0";
+        f << "\n";
+    }
+    f << "        </DataArray>\n      </PointData>\n      <CellData>\n      </CellData>\n    </Piece>\n";
+    f << "  </ImageData>\n</VTKFile>\n";
+}
+
+} // namespace io
+} // namespace m3d

@@ -35,7 +35,7 @@ EXPORTS = [
     "m3d_version",
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
-    "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop",
+    "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop", "m3d_elastic_get_state",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
@@ -88,6 +88,7 @@ def lib():
             "m3d_elastic_last_error": (cp, [vp]),
             "m3d_elastic_profile": (ip, [vp, ip, ip, dp, ipp]),
             "m3d_elastic_get_prop": (ip, [vp, dp, ipp]),
+            "m3d_elastic_get_state": (ip, [vp, dp, dp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -214,6 +215,13 @@ class FFT_Analysis3D:
         out = np.zeros(9 * max(rows.value, 1))
         self._chk(lib().m3d_elastic_get_prop(self._h, _dp(out), ct.byref(rows)))
         return out[: 9 * rows.value].reshape(9, rows.value).T.copy()
+
+    def history(self):
+        """(epsilon_p_eff or d, sigma_eff) per voxel after an "ep"/"ed" runAnalysis."""
+        a = np.empty(self.xyz)
+        b = np.empty(self.xyz)
+        self._chk(lib().m3d_elastic_get_state(self._h, _dp(a), _dp(b)))
+        return a, b
 
     def profile_kernels(self, precision="fp32_polish", reps=10):
         """Average ms per launch of the per-iteration kernels (HIP events on the

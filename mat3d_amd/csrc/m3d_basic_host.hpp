@@ -302,6 +302,18 @@ struct BasicRunner {
         if (eps) HCHK(hipMemcpy(eps, f6, sizeof(double) * n6, hipMemcpyDeviceToHost));
         if (sig) HCHK(hipMemcpy(sig, f6 + n6, sizeof(double) * n6, hipMemcpyDeviceToHost));
     }
+
+    // per-voxel history: equivalent plastic strain ("ep") or damage ("ed"),
+    // and the von Mises stress of the last return map ("ep"; zeros for "ed")
+    void state(double* ep_or_d, double* seff)
+    {
+        const size_t n = (size_t)e.N;
+        if (ep_or_d) HCHK(hipMemcpy(ep_or_d, f1, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (seff) {
+            if (beh == BEH_EP) HCHK(hipMemcpy(seff, f1 + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+            else std::fill(seff, seff + n, 0.0);
+        }
+    }
 };
 
 } // namespace

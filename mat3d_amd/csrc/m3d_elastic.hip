@@ -266,6 +266,18 @@ int m3d_elastic_get_fields(m3d_elastic* h, int lc, double* eps, double* sig)
     });
 }
 
+// public members epsilon_p_eff / sigma_eff ("ep") or d ("ed") of the last
+// runAnalysis, FFT_analysis3D.hpp:56-98 (read by printer and vtiWriter,
+// FFT_analysis3D.cpp:1001-1104)
+int m3d_elastic_get_state(m3d_elastic* h, double* ep_eff_or_d, double* sig_eff)
+{
+    return guarded(h, [&]() {
+        if (!h->basic || !h->last_basic) throw ArgError{M3D_ERR_STATE, "get_state: no \"ep\"/\"ed\" run"};
+        h->basic->state(ep_eff_or_d, sig_eff);
+        return M3D_OK;
+    });
+}
+
 int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, int* geom9)
 {
     return guarded(h, [&]() {

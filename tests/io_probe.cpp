@@ -1,0 +1,65 @@
+// io_probe.cpp -- drives include/m3d_io.hpp for tests/test_io.py (no GPU,
+// no library: the format code only).
+//   io_probe matrix                 a fixed 3x3 matrix in the Eigen stream format
+//   io_probe deck <in_struct.txt>   the parsed fields of a structural deck
+//   io_probe tdeck <in_therm.txt>   the parsed fields of a thermal deck
+//   io_probe report_e < spec        elastic report (spec: Nx Ny Nz strainBase nph,
+//                                   El[nph] v[nph] vol[nph], C S epHomo sigHomo)
+//   io_probe report_t < spec        thermal report (spec: Nx Ny Nz km ki rve_sum, K R G Q)
+//   io_probe vti <file>             a 2x3x4 image, components (i, -i, 0.5 i)
+#include <cstring>
+#include <iostream>
+#include <vector>
+
+#include "m3d_io.hpp"
+
+using namespace m3d::io;
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    if (mode == "matrix") {
+        const double a[9] = {1.5, -2, 3e6, 0.25, 1, 0, 123456789, 0.001, -7};
+        print_matrix(std::cout, a, 3, 3);
+        std::cout << std::endl;
+    } else if (mode == "deck") {
+        const StructDeck d = read_struct_deck(argv[2]);
+        std::cout << d.N << " " << d.E0 << " " << d.v0 << " " << d.E1 << " " << d.v1 << " " << d.tol_field << " "
+                  << d.tol_eq << "\n" << d.rve << " " << d.write_vti << " " << d.vti << " " << d.homogenize << " "
+                  << d.report << "\n";
+        for (double e : d.E) std::cout << e << " ";
+        std::cout << "\n";
+    } else if (mode == "tdeck") {
+        const ThermDeck d = read_therm_deck(argv[2]);
+        std::cout << d.N << " " << d.km << " " << d.ki << " " << d.tol_field << " " << d.tol_eq << "\n"
+                  << d.rve << " " << d.write_vti << " " << d.vti << " " << d.homogenize << " " << d.report << "\n";
+    } else if (mode == "report_e") {
+        ElasticReport r;
+        int sb, n;
+        std::cin >> r.Nx >> r.Ny >> r.Nz >> sb >> n;
+        r.strainBase = sb != 0;
+        r.El.resize(n); r.v.resize(n); r.vol.resize(n);
+        for (auto& x : r.El) std::cin >> x;
+        for (auto& x : r.v) std::cin >> x;
+        for (auto& x : r.vol) std::cin >> x;
+        std::vector<double> m(144);
+        for (auto& x : m) std::cin >> x;
+        r.CHomo = m.data(); r.SHomo = m.data() + 36; r.epHomo = m.data() + 72; r.sigHomo = m.data() + 108;
+        write_report(std::cout, r);
+    } else if (mode == "report_t") {
+        ThermalReport r;
+        std::cin >> r.Nx >> r.Ny >> r.Nz >> r.km >> r.ki >> r.rve_sum;
+        std::vector<double> m(36);
+        for (auto& x : m) std::cin >> x;
+        r.KHomo = m.data(); r.RHomo = m.data() + 9; r.tempGradHomo = m.data() + 18; r.heatFluxHomo = m.data() + 27;
+        write_report(std::cout, r);
+    } else if (mode == "vti") {
+        std::vector<double> a(24 * 2), b(24);
+        for (int i = 0; i < 24; ++i) { a[2 * i] = i; a[2 * i + 1] = -i; b[i] = 0.5 * i; }
+        write_vti(argv[2], 2, 3, 4, 4, {{a.data(), 2}, {b.data(), 1}});
+    } else {
+        return 2;
+    }
+    return 0;
+}
