@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<double> p)
     const cx<T>* base = p.S12 + ((long long)y * g.Nzh + kz0) * C;
     const long long xstride = (long long)g.Ny * g.Nzh * C;
     for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
-    fast::rows_to_lds<T, L, W, NT, false>(sl, base, xstride, nk * C);
+    fast::rows_to_lds<T, L, W, NT, false>(sl, base, fast::uniform_rows(xstride), nk * C);
     __syncthreads();
     fast::fft<T, L, W, NT, -1, 1>(sl, tw);
 

@@ -39,6 +39,12 @@ struct BasicRunner {
         plan_tiles(g12, sizeof(cx<double>));
         plan_tiles(g6, sizeof(cx<double>));
         g12.nsolve = g6.nsolve = 1;
+        // the basic scheme's spectra keep plain [x][y][kz] rows without padding
+        for (Geo* q : {&g12, &g6}) {
+            q->lxb = q->xbm = 0;
+            q->sy = q->sxb = (long long)q->Nzh * q->C;
+            q->sxa = (long long)q->Ny * q->sy;
+        }
         fast = bfast::supported(e.Nx, e.Ny, e.Nz);
         if (fast) {
             gf = g12;

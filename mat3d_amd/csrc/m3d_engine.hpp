@@ -27,6 +27,16 @@ namespace {
 
 constexpr size_t LDS_MAX = 65536;
 
+// spectrum layout of the specialised kernels: x planes per block (power of
+// two; 1 = plain [x][y][kz] rows) and, for plain rows, extra complex values
+// between x planes
+#ifndef M3D_XBLOCK
+#define M3D_XBLOCK 1
+#endif
+#ifndef M3D_XPAD
+#define M3D_XPAD 128
+#endif
+
 struct HipError {
     hipError_t e;
     const char* what;
@@ -163,6 +173,24 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
         g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
         g.nbx = Ny * g.nkx;
     }
+    // spectrum row layout (the runtime-planned kernels address rows (x, y)
+    // at (x * Ny + y) * Nzh * C: plain rows without padding)
+    const long long rowc = (long long)g.Nzh * C;
+    g.lxb = 0;
+    g.xbm = 0;
+    g.sxb = rowc;
+    g.sy = rowc;
+    g.sxa = (long long)Ny * rowc;
+    const bool all_fast = g.fz && g.fy && g.fx;
+    if (all_fast && M3D_XBLOCK > 1 && Nx % M3D_XBLOCK == 0) {
+        while ((1 << g.lxb) < M3D_XBLOCK) ++g.lxb;
+        g.xbm = M3D_XBLOCK - 1;
+        g.sxa = (long long)Ny * M3D_XBLOCK * rowc;
+        g.sy = (long long)M3D_XBLOCK * rowc;
+    } else if (all_fast) {
+        g.sxa += M3D_XPAD; // x planes M3D_XPAD complex values apart beyond their rows
+    }
+    g.spec = (long long)((Nx - 1) >> g.lxb) * g.sxa + (long long)((Nx - 1) & g.xbm) * g.sxb + (long long)(Ny - 1) * g.sy + rowc;
     return g;
 }
 
@@ -342,10 +370,18 @@ struct Engine {
 
     int ring_needed() const { return std::max(2, rollback + 1); }
 
+    // bytes of one solve's spectrum buffer: room for either precision's layout
+    size_t spec_bytes() const
+    {
+        const size_t b32 = (size_t)g32.spec * sizeof(cx<float>), b64 = (size_t)g64.spec * sizeof(cx<double>);
+        const size_t b = std::max(b32, b64);
+        return (b + 255) / 256 * 256;
+    }
+
     // device bytes one solve adds (fields, spectra), for batch sizing
     size_t bytes_per_solve() const
     {
-        const size_t nc = (size_t)N * C, hc = (size_t)H * C;
+        const size_t nc = (size_t)N * C, hc = spec_bytes() / sizeof(cx<double>);
         size_t b = sizeof(double) * nc * 2 + 2 * sizeof(cx<double>) * hc;
         if (precision == M3D_FP32_POLISH) b += sizeof(float) * nc * (ring_needed() + 1);
         return b;
@@ -356,7 +392,7 @@ struct Engine {
         if (n <= cap_solves && (precision != M3D_FP32_POLISH || ring32 == ring_needed())) return;
         free_solves();
         const size_t nc = (size_t)N * C;
-        const size_t hc = (size_t)H * C;
+        const size_t hc = spec_bytes() / sizeof(cx<double>);
         if (precision == M3D_FP32_POLISH) {
             ring32 = ring_needed();
             alloc(&U32, sizeof(float) * nc * n * ring32);
@@ -401,7 +437,7 @@ struct Engine {
         p.sB = (long long)nc * cap_solves;
         p.S = reinterpret_cast<cx<T>*>(Sbuf);
         p.K = reinterpret_cast<cx<T>*>(Kbuf);
-        p.sS = (long long)H * C * (sizeof(cx<double>) / sizeof(cx<T>));
+        p.sS = (long long)(spec_bytes() / sizeof(cx<T>));
         p.ph = ph;
         p.sPh = N;
         p.rve_of = rve_of;

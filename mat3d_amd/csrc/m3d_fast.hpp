@@ -277,9 +277,19 @@ __device__ __forceinline__ void vstore(T* __restrict__ g, const T (&r)[n])
 // issued before any LDS store so the workgroup keeps ROWS*W*sizeof values
 // in flight; pairs of complex values move as one 16-byte access when the
 // layout allows it (VEC).  Entries j >= wv are zero-filled / not stored.
+// offset (complex values) of row r of a tile: (r >> sh) * a + (r & mask) * b
+struct RowMap {
+    long long a, b;
+    int sh, mask;
+    __device__ __forceinline__ long long operator()(int r) const
+    {
+        return (long long)(r >> sh) * a + (long long)(r & mask) * b;
+    }
+};
+__device__ __forceinline__ RowMap uniform_rows(long long stride) { return RowMap{stride, 0, 0, 0}; }
+
 template <typename T, int ROWS, int W, int NT, bool VEC>
-__device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, long long stride,
-                                            int wv)
+__device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, RowMap rm, int wv)
 {
     using V = stage_t<T, VEC>;
     constexpr int G = VEC ? 2 : 1;
@@ -292,7 +302,7 @@ __device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>*
         const int e = threadIdx.x + i * NT;
         if (N % NT == 0 || e < N) {
             const int row = e / WG, j = (e - row * WG) * G;
-            r[i] = j < wv ? *reinterpret_cast<const V*>(base + row * stride + j) : V(0);
+            r[i] = j < wv ? *reinterpret_cast<const V*>(base + rm(row) + j) : V(0);
         }
     }
 #pragma unroll
@@ -306,8 +316,7 @@ __device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>*
 }
 
 template <typename T, int ROWS, int W, int NT, bool VEC>
-__device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>* __restrict__ base, long long stride,
-                                            int wv)
+__device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>* __restrict__ base, RowMap rm, int wv)
 {
     using V = stage_t<T, VEC>;
     constexpr int G = VEC ? 2 : 1;
@@ -319,15 +328,15 @@ __device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>*
         const int e = threadIdx.x + i * NT;
         if (N % NT == 0 || e < N) {
             const int row = e / WG, j = (e - row * WG) * G;
-            if (j < wv) *reinterpret_cast<V*>(base + row * stride + j) = *reinterpret_cast<const V*>(sl + row * W + j);
+            if (j < wv) *reinterpret_cast<V*>(base + rm(row) + j) = *reinterpret_cast<const V*>(sl + row * W + j);
         }
     }
 }
 
-// contiguous spectrum chunk [yl][kz][c] <-> LDS sl[kz*W + yl*C + c]; pairs
-// (c, c+1) move as one 16-byte access for even C in FP32.  The load is split
-// in an issue step (global -> registers) and a commit step (registers ->
-// LDS) so other loads can be issued in between.
+// spectrum chunk of TY rows [yl][kz][c] (row yl at yl * RS) <-> LDS
+// sl[kz*W + yl*C + c]; pairs (c, c+1) move as one 16-byte access for even C
+// in FP32.  The load is split in an issue step (global -> registers) and a
+// commit step (registers -> LDS) so other loads can be issued in between.
 template <typename T, int TY, int NZH, int C, int NT>
 struct ZChunk {
     static constexpr int W = TY * C;
@@ -338,14 +347,19 @@ struct ZChunk {
     using V = stage_t<T, VEC>;
 };
 
-template <typename Z, typename V>
-__device__ __forceinline__ void zchunk_issue(V (&r)[Z::PER], const void* __restrict__ src)
+template <typename Z, int NZH, int C, typename T, typename V>
+__device__ __forceinline__ void zchunk_issue(V (&r)[Z::PER], const cx<T>* __restrict__ src, long long RS)
 {
     constexpr int NT = ZNT;
 #pragma unroll
     for (int i = 0; i < Z::PER; ++i) {
         const int e = threadIdx.x + i * NT;
-        if (Z::N % NT == 0 || e < Z::N) r[i] = reinterpret_cast<const V*>(src)[e];
+        if (Z::N % NT == 0 || e < Z::N) {
+            const int f = e * Z::G;
+            const int yl = f / (NZH * C);
+            const int rr = f - yl * (NZH * C);
+            r[i] = *reinterpret_cast<const V*>(src + yl * RS + rr);
+        }
     }
 }
 
@@ -367,7 +381,7 @@ __device__ __forceinline__ void zchunk_commit(const V (&r)[Z::PER], cx<T>* __res
 }
 
 template <typename T, int TY, int NZH, int C, int NT>
-__device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T>* __restrict__ dst)
+__device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T>* __restrict__ dst, long long RS)
 {
     constexpr int W = TY * C;
     constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
@@ -383,7 +397,7 @@ __device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T
             const int yl = f / (NZH * C);
             const int rr = f - yl * (NZH * C);
             const int kz = rr / C, c = rr - kz * C;
-            reinterpret_cast<V*>(dst)[e] = *reinterpret_cast<const V*>(sl + kz * W + yl * C + c);
+            *reinterpret_cast<V*>(dst + yl * RS + rr) = *reinterpret_cast<const V*>(sl + kz * W + yl * C + c);
         }
     }
 }
@@ -420,7 +434,9 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
     const int tid = threadIdx.x;
 
     const long long vox0 = ((long long)x * g.Ny + y0) * NZ;
-    const long long spec0 = ((long long)x * g.Ny + y0) * NZH * C;
+    // first spectrum row (x, y0) of the tile and the row stride
+    const long long RS = g.sy;
+    const long long spec0 = (long long)(x >> g.lxb) * g.sxa + (long long)(x & g.xbm) * g.sxb + y0 * RS;
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh + vox0;
     const T* coef = p.coef + (long long)rv * p.sCoef;
@@ -448,7 +464,7 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
     using ZC = ZChunk<T, TY, NZH, C, NT>;
     typename ZC::V chunk[ZC::PER];
-    if (IUF) zchunk_issue<ZC>(chunk, p.S + (long long)s * p.sS + spec0);
+    if (IUF) zchunk_issue<ZC, NZH, C>(chunk, p.S + (long long)s * p.sS + spec0, RS);
     T u[QI][2 * C];
     T rr[MODE == MODE_IUFR ? QI : 1][2 * C];
     uint16_t pp[QI];
@@ -585,7 +601,7 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
     __syncthreads();
     {
         cx<T>* dst = ((MODE == MODE_FWD || IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
-        lds_to_zchunk<T, TY, NZH, C, NT>(sl, dst);
+        lds_to_zchunk<T, TY, NZH, C, NT>(sl, dst, RS);
     }
 }
 
@@ -618,21 +634,25 @@ __global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo 
     const int wv = nk * C;
     const int tid = threadIdx.x;
 
+    // spectrum row (x, y) at (x >> lxb) * sxa + (x & xbm) * sxb + y * sy
     cx<T>* base;
-    long long estride;
+    RowMap rm;
     const cx<T>* twg;
     if (KIND == 0) {
-        base = (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + ((long long)o * g.Ny * g.Nzh + kz0) * C;
-        estride = (long long)g.Nzh * C;
+        // tile: plane x = o, lines along y
+        base = (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + (long long)(o >> g.lxb) * g.sxa +
+               (long long)(o & g.xbm) * g.sxb + (long long)kz0 * C;
+        rm = RowMap{0, g.sy, 30, (1 << 30) - 1};
         twg = p.twy;
     } else {
-        base = (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + ((long long)o * g.Nzh + kz0) * C;
-        estride = (long long)g.Ny * g.Nzh * C;
+        // tile: row y = o, lines along x
+        base = (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + (long long)o * g.sy + (long long)kz0 * C;
+        rm = RowMap{g.sxa, g.sxb, g.lxb, g.xbm};
         twg = p.twx;
     }
     for (int i = tid; i < L; i += NT) tw[i] = twg[i];
     constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
-    rows_to_lds<T, L, W, NT, VEC>(sl, base, estride, wv);
+    rows_to_lds<T, L, W, NT, VEC>(sl, base, rm, wv);
     __syncthreads();
     fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1>(sl, tw);
 
@@ -685,7 +705,7 @@ __global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo 
         if (tid == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + t] = bs;
         return;
     }
-    lds_to_rows<T, L, W, NT, VEC>(sl, base, estride, wv);
+    lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
 }
 
 } // namespace fast

@@ -85,6 +85,14 @@ struct Geo {
     int nsolve;
     int nbz, nbx;       // blocks per solve of the z pass and the x pass
     double invN;
+    // spectrum layout of the specialised kernels: the half-spectrum row
+    // (x, y) (Nzh*C complex values) starts at
+    //   (x >> lxb) * sxa + (x & xbm) * sxb + y * sy
+    // plain rows: lxb = 0, xbm = 0, sxa = Ny*Nzh*C (+ padding), sy = Nzh*C;
+    // x planes in blocks of 2^lxb: sxa = Ny*2^lxb*Nzh*C, sxb = Nzh*C, sy = 2^lxb*Nzh*C
+    int lxb, xbm;
+    long long sxa, sxb, sy;
+    long long spec;     // complex values of one solve's spectrum (rows + padding)
     int fz, fy, fx;     // host only: compile-time specialised kernel used for this axis
     int gen;            // host only: generic kernels need the odd radices
 };
