@@ -162,6 +162,7 @@ def main():
         d.close()
 
     # per-kernel timings (HIP events on the solver stream) and roofline
+    tuning = s.tuning()
     ms, geom = s.profile_kernels("fp32_polish", reps=args.profile_reps)
     kb = kernel_bytes(geom, 4)
     kernels = {}
@@ -213,6 +214,7 @@ def main():
         "roofline_iteration": {"bytes_per_voxel_iter": round(B, 2), "ms_per_iter_batch": round(it_ms, 4),
                                "achieved": round(it_gbs, 1), "frac": round(it_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "autotune": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -134,6 +134,13 @@ int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
  * {Nx, Ny, Nz, Nz/2+1, C, solves, TY, TKy, TKx}.  Overwrites the fields. */
 int m3d_elastic_profile(m3d_elastic *h, int precision, int reps, double *ms6, int *geom9);
 
+/* Variants of the FP32 x/y passes the autotuner chose for the last solve's
+ * shape (no reference counterpart): threads per workgroup of the x pass with
+ * the Green step, its wide kz tile (1) or not (0), threads of the y pass.
+ * Option "autotune" (default 1) times the variants at the first FP32 solve
+ * of a (grid, solves) shape. */
+int m3d_elastic_get_tuning(m3d_elastic *h, int *xg_threads, int *xg_wide, int *y_threads);
+
 const char *m3d_elastic_last_error(m3d_elastic *h);
 
 /* ---------------------------------------------------------------- thermal */

@@ -36,6 +36,7 @@ EXPORTS = [
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
     "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop", "m3d_elastic_get_state",
+    "m3d_elastic_get_tuning",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
@@ -89,6 +90,7 @@ def lib():
             "m3d_elastic_profile": (ip, [vp, ip, ip, dp, ipp]),
             "m3d_elastic_get_prop": (ip, [vp, dp, ipp]),
             "m3d_elastic_get_state": (ip, [vp, dp, dp]),
+            "m3d_elastic_get_tuning": (ip, [vp, ipp, ipp, ipp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -222,6 +224,12 @@ class FFT_Analysis3D:
         b = np.empty(self.xyz)
         self._chk(lib().m3d_elastic_get_state(self._h, _dp(a), _dp(b)))
         return a, b
+
+    def tuning(self):
+        """Autotuned variants of the FP32 x/y passes for the last solve's shape."""
+        a, b, c = ct.c_int(0), ct.c_int(0), ct.c_int(0)
+        self._chk(lib().m3d_elastic_get_tuning(self._h, ct.byref(a), ct.byref(b), ct.byref(c)))
+        return {"xgreen_threads": a.value, "xgreen_wide_tile": b.value, "ypass_threads": c.value}
 
     def profile_kernels(self, precision="fp32_polish", reps=10):
         """Average ms per launch of the per-iteration kernels (HIP events on the

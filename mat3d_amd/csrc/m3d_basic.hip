@@ -312,8 +312,8 @@ bool y_at(const Geo& g, const Ptr<double>& p, hipStream_t st)
     constexpr int TK = fast::xytile<double>(L, PH::C);
     const int nkt = (g.Nzh + TK - 1) / TK;
     const size_t sm = ((size_t)L * TK * PH::C + L) * sizeof(cx<double>);
-    hipLaunchKernelGGL((fast::kxy<double, PH, 0, DIR, WHICH, L, TK>), dim3(g.Nx * nkt),
-                       dim3(fast::xy_threads<0, double>()), sm, st, g, p, (const double*)nullptr);
+    hipLaunchKernelGGL((fast::kxy<double, PH, 0, DIR, WHICH, L, TK, 256>), dim3(g.Nx * nkt), dim3(256), sm, st, g, p,
+                       (const double*)nullptr);
     return true;
 }
 

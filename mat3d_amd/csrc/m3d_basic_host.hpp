@@ -12,6 +12,7 @@ struct BasicRunner {
     Engine<Elastic>& e;
     Geo g12{}, g6{};
     Geo gf{};          // tiles of the compile-time specialised kernels
+    Geo gf6{};         // the same with the row strides of the 6-component spectrum
     bool fast = false; // grid has specialised basic kernels (m3d_basic.hip)
     int beh;
     double* f6 = nullptr;   // eps, sig, sig_old, eps_old (4 x 6N)
@@ -54,6 +55,9 @@ struct BasicRunner {
             gf.TKx = bfast::xtile(e.Nx);
             gf.nkx = (gf.Nzh + gf.TKx - 1) / gf.TKx;
             gf.nbx = e.Ny * gf.nkx;
+            gf6 = gf;
+            gf6.sy = gf6.sxb = (long long)gf.Nzh * 6;
+            gf6.sxa = (long long)gf.Ny * gf6.sy;
         }
         const int nbz = fast ? std::max(g12.nbz, gf.nbz) : g12.nbz;
         const int nbx = fast ? std::max(g12.nbx, gf.nbx) : g12.nbx;
@@ -167,7 +171,7 @@ struct BasicRunner {
         bfast::launch_y12(gf, yp, st);
         bfast::launch_x(gf, bp, st);
         hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, pchk);
-        bfast::launch_y6(gf, yp, st);
+        bfast::launch_y6(gf6, yp, st);
     }
 
     void run_stage(int mode)

@@ -43,7 +43,7 @@ struct m3d_batch {
     bool grouped = false;
     bool ran = false;
     double polish_factor = 10.0, fp32_floor = 1e-6;
-    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1, scheme = 0;
+    int rollback = 5, cap = 2000, chunk = 0, max_solves = 0, refine = 1, scheme = 0, autotune = 1;
     double ms = 0, voxel_its = 0;
 };
 
@@ -130,6 +130,7 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
         else if (k == "cap" && value >= 1) h->eng->cap = (int)value;
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
         else if (k == "refine" && (value == 0 || value == 1)) h->eng->refine = (int)value;
+        else if (k == "autotune" && (value == 0 || value == 1)) h->eng->autotune = (int)value;
         else if (k == "scheme" && (value == 0 || value == 1)) {
             h->scheme = (int)value;
             if (h->have_mat) {
@@ -278,6 +279,16 @@ int m3d_elastic_get_state(m3d_elastic* h, double* ep_eff_or_d, double* sig_eff)
     });
 }
 
+int m3d_elastic_get_tuning(m3d_elastic* h, int* xg_threads, int* xg_wide, int* y_threads)
+{
+    return guarded(h, [&]() {
+        if (xg_threads) *xg_threads = h->eng->tune32.xg_nt;
+        if (xg_wide) *xg_wide = h->eng->tune32.xg_wide;
+        if (y_threads) *y_threads = h->eng->tune32.y_nt;
+        return M3D_OK;
+    });
+}
+
 int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, int* geom9)
 {
     return guarded(h, [&]() {
@@ -335,6 +346,7 @@ int m3d_batch_set_option(m3d_batch* b, const char* key, double value)
             batch_free_groups(b); // device batch sizes change
         }
         else if (k == "refine" && (value == 0 || value == 1)) b->refine = (int)value;
+        else if (k == "autotune" && (value == 0 || value == 1)) b->autotune = (int)value;
         else if (k == "scheme" && (value == 0 || value == 1)) {
             b->scheme = (int)value;
             for (BatchGroup& G : b->groups) G.loaded = false; // coefficient tables change
@@ -408,6 +420,7 @@ int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
             e.cap = b->cap;
             e.chunk = b->chunk;
             e.refine = b->refine;
+            e.autotune = b->autotune;
             const std::vector<int>& members = G.members;
             for (size_t k0 = 0; k0 < members.size(); k0 += (size_t)G.per) {
                 const size_t k1 = std::min(members.size(), k0 + (size_t)G.per);

@@ -11,6 +11,9 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -249,6 +252,34 @@ void max_min(const std::vector<double>& a, double& mx, int& imx, double& mn)
 }
 
 // --------------------------------------------------------------- engine
+// ------------------------------------------------------------------ autotuner
+// Variants of the FP32 y pass and x pass with the Green step: threads per
+// workgroup (256 / 512) and, for the x pass, the kz tile (xtile / 1.5x
+// wider).  Each is timed on the live buffers of the first FP32 solve of a
+// (grid, components, solves) shape; the choice is kept for the process.
+struct XyTune {
+    int xg_nt = 512, xg_wide = 0, y_nt = 512;
+};
+
+struct TuneKey {
+    int nx, ny, nz, c, ns;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(nx, ny, nz, c, ns) < std::tie(o.nx, o.ny, o.nz, o.c, o.ns);
+    }
+};
+
+inline std::map<TuneKey, XyTune>& tune_cache()
+{
+    static std::map<TuneKey, XyTune> m;
+    return m;
+}
+inline std::mutex& tune_mutex()
+{
+    static std::mutex mu;
+    return mu;
+}
+
 template <typename PH>
 struct Engine {
     static constexpr int C = PH::C;
@@ -286,6 +317,10 @@ struct Engine {
     std::vector<m3d_run_info> info;
     double polish_factor = 10.0;
     int refine = 1; // anchored FP32 polish (1) or plain FP64 polish iterations (0)
+    int autotune = 1; // time the x/y pass variants at the first FP32 solve of a shape
+    XyTune tune32;    // variants of the FP32 x/y passes in use
+    bool tuned32 = false;
+    int tuned_ns = 0; // batch size the variants were timed at
     int rollback = 5;
     double rollback_margin = 2.0;
     double fp32_floor = 1e-6;
@@ -494,26 +529,48 @@ struct Engine {
         else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
 
-    // KIND 0: y pass, 1: x pass + Green, 2: x pass of the check FFT
+    template <typename T, int KIND, int DIR, int WHICH, int L, int TK, int NTH>
+    void xy_launch(const Geo& g, const Ptr<T>& p)
+    {
+        const int nkt = (g.Nzh + TK - 1) / TK;
+        const int outer = KIND == 0 ? g.Nx : g.Ny;
+        const dim3 grid(g.nsolve * outer * nkt);
+        const size_t sm = ((size_t)L * TK * C + L) * sizeof(cx<T>);
+        if (sm > 65536) {
+            // gfx950: up to 160 KB of LDS per workgroup
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        }
+        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH>), grid, dim3(NTH), sm, st, g, p, prm<T>());
+    }
+
+    // KIND 0: y pass, 1: x pass + Green, 2: x pass of the check FFT.  The FP32
+    // y pass and x pass with the Green step run the variant the autotuner
+    // picked (threads per workgroup, kz tile width); the rest one shape.
     template <typename T, int KIND, int DIR, int WHICH, int L>
     bool xyf(const Geo& g, const Ptr<T>& p)
     {
         const int Lax = KIND == 0 ? g.Ny : g.Nx;
         if (Lax != L) return false;
-        const int TK = KIND == 0 ? g.TKy : g.TKx;
         constexpr int TKc = KIND == 0 ? fast::xytile<T>(L, C) : fast::xtile<T>(L, C);
-        const int nkt = (g.Nzh + TK - 1) / TK;
-        const int outer = KIND == 0 ? g.Nx : g.Ny;
-        const dim3 grid(g.nsolve * outer * nkt);
+        const int TK = KIND == 0 ? g.TKy : g.TKx;
         if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
-        const size_t sm = ((size_t)L * TKc * C + L) * sizeof(cx<T>);
-        if (sm > 65536) {
-            // gfx950: up to 160 KB of LDS per workgroup
-            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        if constexpr (sizeof(T) == 4 && KIND == 1) {
+            constexpr int TKw = fast::xtile_wide<T>(L, C);
+            const XyTune& tu = tune32;
+            if (tu.xg_wide) {
+                if (tu.xg_nt == 512) xy_launch<T, KIND, DIR, WHICH, L, TKw, 512>(g, p);
+                else xy_launch<T, KIND, DIR, WHICH, L, TKw, 256>(g, p);
+            } else {
+                if (tu.xg_nt == 512) xy_launch<T, KIND, DIR, WHICH, L, TKc, 512>(g, p);
+                else xy_launch<T, KIND, DIR, WHICH, L, TKc, 256>(g, p);
+            }
+        } else if constexpr (sizeof(T) == 4 && KIND == 0) {
+            if (tune32.y_nt == 512) xy_launch<T, KIND, DIR, WHICH, L, TKc, 512>(g, p);
+            else xy_launch<T, KIND, DIR, WHICH, L, TKc, 256>(g, p);
+        } else {
+            xy_launch<T, KIND, DIR, WHICH, L, TKc, fast::xy_threads<KIND, T>()>(g, p);
         }
-        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TKc>), grid, dim3(fast::xy_threads<KIND, T>()), sm, st, g, p,
-                           prm<T>());
         return true;
     }
 
@@ -689,6 +746,66 @@ struct Engine {
         run_graph(chunk_for(gf), [&]() { launch_iter_corr(gf, pf, gd, pd); }, hctl);
     }
 
+    // time the FP32 x/y pass variants on this shape (first FP32 solve); the
+    // kernels run on the live buffers, whose contents the solve then rewrites
+    void tune_xy(Geo g)
+    {
+        g.nsolve = nsolve;
+        const TuneKey key{Nx, Ny, Nz, C, nsolve};
+        {
+            std::lock_guard<std::mutex> lk(tune_mutex());
+            auto it = tune_cache().find(key);
+            if (it != tune_cache().end()) {
+                tune32 = it->second;
+                tuned32 = true;
+                return;
+            }
+        }
+        Ptr<float> p = ptrs<float>(g);
+        auto time_it = [&](auto&& launch) {
+            launch();
+            HCHK(hipEventRecord(ev0, st));
+            for (int r = 0; r < 3; ++r) launch();
+            HCHK(hipEventRecord(ev1, st));
+            HCHK(hipEventSynchronize(ev1));
+            float ms = 0;
+            HCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            return ms;
+        };
+        XyTune best = tune32;
+        float bt = 1e30f;
+        if (g.fx) {
+            for (int nt : {256, 512})
+                for (int wide : {0, 1}) {
+                    tune32.xg_nt = nt;
+                    tune32.xg_wide = wide;
+                    const float t = time_it([&]() { xgreen<float>(g, p); });
+                    if (t < bt) {
+                        bt = t;
+                        best.xg_nt = nt;
+                        best.xg_wide = wide;
+                    }
+                }
+        }
+        tune32 = best;
+        bt = 1e30f;
+        if (g.fy) {
+            for (int nt : {256, 512}) {
+                tune32.y_nt = nt;
+                const float t = time_it([&]() { ypass<float, -1, BUF_MAIN>(g, p); });
+                if (t < bt) {
+                    bt = t;
+                    best.y_nt = nt;
+                }
+            }
+        }
+        HCHK(hipGetLastError());
+        tune32 = best;
+        tuned32 = true;
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_cache()[key] = best;
+    }
+
     // loads: nsolve x 6 (only the first C entries used); tol per phase
     // loads, norms, base flags and RVE indices per solve (sb/rve empty: the
     // handle's single RVE with base flag sb0)
@@ -716,6 +833,10 @@ struct Engine {
             c.tol_eq = two ? std::max(tol_eq * polish_factor, fp32_floor) : tol_eq;
         }
         HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
+        if (precision == M3D_FP32_POLISH && autotune && (!tuned32 || tuned_ns != nsolve)) {
+            tune_xy(g32);
+            tuned_ns = nsolve;
+        }
         final_prec.assign(nsolve, 1);
         info.assign(nsolve, m3d_run_info{});
         HCHK(hipEventRecord(ev0, st));

@@ -109,6 +109,14 @@ constexpr int xtile(int L, int C)
     int tk = budget / (C * L);
     return tk < 1 ? 1 : tk;
 }
+// the autotuner's wide alternative for the x pass with the Green step (1.5x)
+template <typename T>
+constexpr int xtile_wide(int L, int C)
+{
+    int budget = (sizeof(T) == 4 ? M3D_XTILE : M3D_XTILE / 2) * 3 / 2;
+    int tk = budget / (C * L);
+    return tk < 1 ? 1 : tk;
+}
 constexpr int ZNT = 256;
 
 // threads per workgroup of the x/y passes: y pass, x pass with the Green
@@ -607,12 +615,12 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
 
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
-template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK>
-__global__ void __launch_bounds__((xy_threads<KIND, T>())) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>()>
+__global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
-    constexpr int NT = xy_threads<KIND, T>();
+    constexpr int NT = NTH;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[NT / 64];
     cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);

@@ -88,6 +88,7 @@ int m3d_thermal_set_option(m3d_thermal* h, const char* key, double value)
         else if (k == "cap" && value >= 1) h->eng->cap = (int)value;
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
         else if (k == "refine" && (value == 0 || value == 1)) h->eng->refine = (int)value;
+        else if (k == "autotune" && (value == 0 || value == 1)) h->eng->autotune = (int)value;
         else throw ArgError{M3D_ERR_ARG, "unknown option or bad value: " + k};
         return M3D_OK;
     });
