@@ -81,11 +81,14 @@ constexpr int radix(int L, int p)
 constexpr bool supported(int L) { return odd_part(L) == 1 || odd_part(L) == 3 || odd_part(L) == 9; }
 
 // tile shapes (shared with the host planner)
+#ifndef M3D_ZBUDGET
+#define M3D_ZBUDGET 3072 // complex values per z tile (FP32; FP64 half)
+#endif
 template <typename T>
 constexpr int zlines(int LZ, int C, int Ny)
 {
     // TY: largest power of two with TY*C*LZ <= budget that divides Ny
-    int budget = sizeof(T) == 4 ? 3072 : 1536;
+    int budget = sizeof(T) == 4 ? M3D_ZBUDGET : M3D_ZBUDGET / 2;
     int ty = 1;
     while (ty < 16 && ty * 2 * C * LZ <= budget && Ny % (ty * 2) == 0) ty *= 2;
     return ty;
