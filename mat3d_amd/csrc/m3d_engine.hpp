@@ -123,6 +123,8 @@ void plan_tiles(Geo& g, size_t csz)
     g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
     g.nbz = g.Nx * g.nty;
     g.nbx = g.Ny * g.nkx;
+    g.TKc = g.TKx;
+    g.nbc = g.nbx;
 }
 
 // line lengths with compile-time specialised kernels (per physics)
@@ -175,6 +177,8 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
         g.TKx = fast::xtile<T>(Nx, C);
         g.nkx = (g.Nzh + g.TKx - 1) / g.TKx;
         g.nbx = Ny * g.nkx;
+        g.TKc = fast::xctile<T>(Nx, C);
+        g.nbc = Ny * ((g.Nzh + g.TKc - 1) / g.TKc);
     }
     // spectrum row layout (the runtime-planned kernels address rows (x, y)
     // at (x * Ny + y) * Nzh * C: plain rows without padding)
@@ -436,7 +440,7 @@ struct Engine {
         alloc(&U64, sizeof(double) * nc * n * 2);
         alloc(&Sbuf, sizeof(cx<double>) * hc * n);
         alloc(&Kbuf, sizeof(cx<double>) * hc * n);
-        const int nbz = std::max(g32.nbz, g64.nbz), nbx = std::max(g32.nbx, g64.nbx);
+        const int nbz = std::max(g32.nbz, g64.nbz), nbx = std::max(g32.nbc, g64.nbc);
         alloc(&psum, sizeof(double) * (size_t)nbz * n);
         alloc(&pchk, sizeof(double) * (size_t)2 * (nbx + 1) * n);
         alloc(&ctl, sizeof(Ctl) * n);
@@ -534,8 +538,8 @@ struct Engine {
     {
         const int nkt = (g.Nzh + TK - 1) / TK;
         const int outer = KIND == 0 ? g.Nx : g.Ny;
-        const dim3 grid(g.nsolve * outer * nkt);
         const size_t sm = ((size_t)L * TK * C + L) * sizeof(cx<T>);
+        const dim3 grid(g.nsolve * outer * nkt);
         if (sm > 65536) {
             // gfx950: up to 160 KB of LDS per workgroup
             HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH>),
@@ -552,8 +556,8 @@ struct Engine {
     {
         const int Lax = KIND == 0 ? g.Ny : g.Nx;
         if (Lax != L) return false;
-        constexpr int TKc = KIND == 0 ? fast::xytile<T>(L, C) : fast::xtile<T>(L, C);
-        const int TK = KIND == 0 ? g.TKy : g.TKx;
+        constexpr int TKc = KIND == 0 ? fast::xytile<T>(L, C) : (KIND == 1 ? fast::xtile<T>(L, C) : fast::xctile<T>(L, C));
+        const int TK = KIND == 0 ? g.TKy : (KIND == 1 ? g.TKx : g.TKc);
         if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
         if constexpr (sizeof(T) == 4 && KIND == 1) {
             constexpr int TKw = fast::xtile_wide<T>(L, C);

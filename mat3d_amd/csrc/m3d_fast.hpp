@@ -93,10 +93,13 @@ constexpr int zlines(int LZ, int C, int Ny)
     while (ty < 16 && ty * 2 * C * LZ <= budget && Ny % (ty * 2) == 0) ty *= 2;
     return ty;
 }
+#ifndef M3D_YTILE
+#define M3D_YTILE 6144
+#endif
 template <typename T>
 constexpr int xytile(int L, int C)
 {
-    int budget = sizeof(T) == 4 ? 6144 : 3072;
+    int budget = sizeof(T) == 4 ? M3D_YTILE : M3D_YTILE / 2;
     int tk = budget / (C * L);
     return tk < 1 ? 1 : tk;
 }
@@ -109,6 +112,18 @@ template <typename T>
 constexpr int xtile(int L, int C)
 {
     int budget = sizeof(T) == 4 ? M3D_XTILE : M3D_XTILE / 2;
+    int tk = budget / (C * L);
+    return tk < 1 ? 1 : tk;
+}
+// x pass of the equilibrium check (FP32): narrower kz tiles, so more
+// workgroups share a CU's LDS while the tile is read (the pass only reads)
+#ifndef M3D_XCTILE
+#define M3D_XCTILE 3072
+#endif
+template <typename T>
+constexpr int xctile(int L, int C)
+{
+    int budget = sizeof(T) == 4 ? M3D_XCTILE : M3D_XTILE / 2;
     int tk = budget / (C * L);
     return tk < 1 ? 1 : tk;
 }
@@ -170,6 +185,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return b;
 }
 
+// thread index as an opaque value: address arithmetic derived from it is
+// redone where it is used instead of being hoisted out of a tile loop and
+// held in registers across it
+__device__ __forceinline__ int lane_tid()
+{
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // --------------------------------------------------------------- passes
 template <int DIR, typename T>
 __device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w)
@@ -190,7 +215,7 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
     constexpr int items = W * nR;
     constexpr int IPT = (items + NT - 1) / NT;
     constexpr int tstep = (L / (Ns * R)) * TWS;
-    const int tid = threadIdx.x;
+    const int tid = lane_tid();
     cx<T> v[IPT][R];
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
@@ -299,31 +324,55 @@ struct RowMap {
 };
 __device__ __forceinline__ RowMap uniform_rows(long long stride) { return RowMap{stride, 0, 0, 0}; }
 
+// a thread's share of a ROWS x W tile, split into an issue step (global ->
+// registers) and a commit step (registers -> LDS)
 template <typename T, int ROWS, int W, int NT, bool VEC>
-__device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, RowMap rm, int wv)
-{
+struct RowTile {
     using V = stage_t<T, VEC>;
-    constexpr int G = VEC ? 2 : 1;
-    constexpr int WG = W / G;
-    constexpr int N = ROWS * WG;
-    constexpr int PER = (N + NT - 1) / NT;
-    V r[PER];
+    static constexpr int G = VEC ? 2 : 1;
+    static constexpr int WG = W / G;
+    static constexpr int N = ROWS * WG;
+    static constexpr int PER = (N + NT - 1) / NT;
+};
+
+template <typename T, int ROWS, int W, int NT, bool VEC, typename V, int PER>
+__device__ __forceinline__ void rows_load(V (&r)[PER], const cx<T>* __restrict__ base, RowMap rm, int wv)
+{
+    using RT = RowTile<T, ROWS, W, NT, VEC>;
+    static_assert(PER == RT::PER, "register tile size");
+    const int t0 = lane_tid();
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * NT;
-        if (N % NT == 0 || e < N) {
-            const int row = e / WG, j = (e - row * WG) * G;
+    for (int i = 0; i < RT::PER; ++i) {
+        const int e = t0 + i * NT;
+        if (RT::N % NT == 0 || e < RT::N) {
+            const int row = e / RT::WG, j = (e - row * RT::WG) * RT::G;
             r[i] = j < wv ? *reinterpret_cast<const V*>(base + rm(row) + j) : V(0);
         }
     }
+}
+
+template <typename T, int ROWS, int W, int NT, bool VEC, typename V, int PER>
+__device__ __forceinline__ void rows_commit(cx<T>* __restrict__ sl, const V (&r)[PER])
+{
+    using RT = RowTile<T, ROWS, W, NT, VEC>;
+    static_assert(PER == RT::PER, "register tile size");
+    const int t0 = lane_tid();
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * NT;
-        if (N % NT == 0 || e < N) {
-            const int row = e / WG, j = (e - row * WG) * G;
+    for (int i = 0; i < RT::PER; ++i) {
+        const int e = t0 + i * NT;
+        if (RT::N % NT == 0 || e < RT::N) {
+            const int row = e / RT::WG, j = (e - row * RT::WG) * RT::G;
             *reinterpret_cast<V*>(sl + row * W + j) = r[i];
         }
     }
+}
+
+template <typename T, int ROWS, int W, int NT, bool VEC>
+__device__ __forceinline__ void rows_to_lds(cx<T>* __restrict__ sl, const cx<T>* __restrict__ base, RowMap rm, int wv)
+{
+    typename RowTile<T, ROWS, W, NT, VEC>::V r[RowTile<T, ROWS, W, NT, VEC>::PER];
+    rows_load<T, ROWS, W, NT, VEC>(r, base, rm, wv);
+    rows_commit<T, ROWS, W, NT, VEC>(sl, r);
 }
 
 template <typename T, int ROWS, int W, int NT, bool VEC>
@@ -334,9 +383,10 @@ __device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>*
     constexpr int WG = W / G;
     constexpr int N = ROWS * WG;
     constexpr int PER = (N + NT - 1) / NT;
+    const int t0 = lane_tid();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * NT;
+        const int e = t0 + i * NT;
         if (N % NT == 0 || e < N) {
             const int row = e / WG, j = (e - row * WG) * G;
             if (j < wv) *reinterpret_cast<V*>(base + rm(row) + j) = *reinterpret_cast<const V*>(sl + row * W + j);
@@ -618,57 +668,35 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
 
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
-template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>()>
-__global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+
+// tile (s, o, kz0) of a y or x pass: its first spectrum row and row map
+template <typename T, int C, int KIND, int WHICH>
+__device__ __forceinline__ const cx<T>* xy_rows(const Geo& g, const Ptr<T>& p, int s, int o, int kz0, RowMap& rm)
+{
+    if (KIND == 0) {
+        // tile: plane x = o, lines along y
+        rm = RowMap{0, g.sy, 30, (1 << 30) - 1};
+        return (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + (long long)(o >> g.lxb) * g.sxa +
+               (long long)(o & g.xbm) * g.sxb + (long long)kz0 * C;
+    }
+    // tile: row y = o, lines along x; row (x, y) at (x >> lxb) * sxa + (x & xbm) * sxb + y * sy
+    rm = RowMap{g.sxa, g.sxb, g.lxb, g.xbm};
+    return (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + (long long)o * g.sy + (long long)kz0 * C;
+}
+
+// a tile already in LDS: forward FFT along the lines, then the Green step and
+// the inverse FFT (KIND 1) or the equilibrium partial sums (KIND 2; t is the
+// tile's index within its solve).  Returns with the tile in LDS (KIND 0, 1).
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NT>
+__device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl& ctl, const T* __restrict__ prm, cx<T>* sl,
+                                        const cx<T>* tw, double* red, int s, int o, int kz0, int nk, int t)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
-    constexpr int NT = NTH;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[NT / 64];
-    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* tw = sl + L * W;
-
-    const int nkt = (g.Nzh + TK - 1) / TK;
-    const int lines_out = KIND == 0 ? g.Nx : g.Ny; // outer index of the tile
-    const int per = lines_out * nkt;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int s = b / per;
-    const int t = b - s * per;
-    const int o = t / nkt;
-    const int kz0 = (t - o * nkt) * TK;
-    const Ctl& ctl = p.ctl[s];
-    if (ctl.done) return;
-    if (WHICH == BUF_CHK_CUR && !ctl.need_cur) return;
-    if (WHICH == BUF_CHK_PREV && !ctl.need_prev) return;
-    const int nk = min(TK, g.Nzh - kz0);
-    const int wv = nk * C;
-    const int tid = threadIdx.x;
-
-    // spectrum row (x, y) at (x >> lxb) * sxa + (x & xbm) * sxb + y * sy
-    cx<T>* base;
-    RowMap rm;
-    const cx<T>* twg;
-    if (KIND == 0) {
-        // tile: plane x = o, lines along y
-        base = (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + (long long)(o >> g.lxb) * g.sxa +
-               (long long)(o & g.xbm) * g.sxb + (long long)kz0 * C;
-        rm = RowMap{0, g.sy, 30, (1 << 30) - 1};
-        twg = p.twy;
-    } else {
-        // tile: row y = o, lines along x
-        base = (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + (long long)o * g.sy + (long long)kz0 * C;
-        rm = RowMap{g.sxa, g.sxb, g.lxb, g.xbm};
-        twg = p.twx;
-    }
-    for (int i = tid; i < L; i += NT) tw[i] = twg[i];
-    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
-    rows_to_lds<T, L, W, NT, VEC>(sl, base, rm, wv);
-    __syncthreads();
+    const int tid = lane_tid();
     fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1>(sl, tw);
 
     if constexpr (KIND == 1) {
-        const T* prm = prm_all + 2 * p.rve_of[s];
         const int y = o;
         for (int f = tid; f < L * nk; f += NT) {
             const int kx = f / nk, kzl = f - kx * nk;
@@ -709,14 +737,57 @@ __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T
                 double dd;
                 if constexpr (C == 6) dd = sqrt(a2[0] + a2[1] + a2[2] + 2 * a2[3] + 2 * a2[4] + 2 * a2[5]);
                 else dd = sqrt(a2[0] + a2[1] + a2[2]);
-                p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + g.nbx] = dd;
+                p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbc + 1) + g.nbc] = dd;
             }
         }
         const double bs = block_sum(acc, red);
-        if (tid == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + t] = bs;
-        return;
+        if (tid == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbc + 1) + t] = bs;
     }
-    lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
+}
+
+// solve s takes part in this pass
+template <int WHICH>
+__device__ __forceinline__ bool xy_active(const Ctl& c)
+{
+    if (c.done) return false;
+    if (WHICH == BUF_CHK_CUR && !c.need_cur) return false;
+    if (WHICH == BUF_CHK_PREV && !c.need_prev) return false;
+    return true;
+}
+
+// one workgroup per tile
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>()>
+__global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+{
+    constexpr int C = PH::C;
+    constexpr int W = TK * C;
+    constexpr int NT = NTH;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NT / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw = sl + L * W;
+
+    const int nkt = (g.Nzh + TK - 1) / TK;
+    const int lines_out = KIND == 0 ? g.Nx : g.Ny; // outer index of the tile
+    const int per = lines_out * nkt;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = b / per;
+    const int t = b - s * per;
+    const int o = t / nkt;
+    const int kz0 = (t - o * nkt) * TK;
+    if (!xy_active<WHICH>(p.ctl[s])) return;
+    const int nk = min(TK, g.Nzh - kz0);
+    const int wv = nk * C;
+
+    RowMap rm;
+    cx<T>* base = const_cast<cx<T>*>(xy_rows<T, C, KIND, WHICH>(g, p, s, o, kz0, rm));
+    const cx<T>* twg = KIND == 0 ? p.twy : p.twx;
+    for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
+    rows_to_lds<T, L, W, NT, VEC>(sl, base, rm, wv);
+    __syncthreads();
+    xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT>(g, p, p.ctl[s], prm_all + 2 * p.rve_of[s], sl, tw, red, s, o, kz0, nk, t);
+    if constexpr (KIND != 2) lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
 }
 
 } // namespace fast

@@ -84,6 +84,7 @@ struct Geo {
     int lsz;            // complex stride of one z line in LDS
     int nsolve;
     int nbz, nbx;       // blocks per solve of the z pass and the x pass
+    int TKc, nbc;       // x pass of the equilibrium check: kz tile, partial sums per solve
     double invN;
     // spectrum layout of the specialised kernels: the half-spectrum row
     // (x, y) (Nzh*C complex values) starts at
@@ -115,7 +116,7 @@ struct Ptr {
     const cx<T>* twy;
     const cx<T>* twx;
     double* psum;           // z-pass partial sums, [nsolve][nbz]
-    double* pchk;           // check partials, [nsolve][2][nbx + 1]
+    double* pchk;           // check partials, [nsolve][2][nbc + 1]
     Ctl* ctl;
     // anchored polish (solves with ctl.corr): FP32 passes add R to the
     // update; FP64 passes read the iterate as U[anc_it] + e, e from the FP32
@@ -705,11 +706,11 @@ __global__ void __launch_bounds__(XYTHREADS) k_xcheck(Geo g, Ptr<T> p)
             double dd;
             if constexpr (C == 6) dd = sqrt(a2[0] + a2[1] + a2[2] + 2 * a2[3] + 2 * a2[4] + 2 * a2[5]);
             else dd = sqrt(a2[0] + a2[1] + a2[2]);
-            p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + g.nbx] = dd;
+            p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbc + 1) + g.nbc] = dd;
         }
     }
     const double bs = block_sum(acc, red);
-    if (threadIdx.x == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbx + 1) + t] = bs;
+    if (threadIdx.x == 0) p.pchk[((long long)s * 2 + (WHICH == BUF_CHK_CUR ? 0 : 1)) * (g.nbc + 1) + t] = bs;
 }
 
 // ---------------------------------------------------------- finalizers
@@ -780,13 +781,13 @@ __global__ void __launch_bounds__(FTHREADS) k_finB(Geo g, Ctl* ctl, const double
     const int done = c.done, need_cur = c.need_cur, need_prev = c.need_prev;
     __syncthreads();
     if (done || !need_cur) return;
-    const double* pc = pchk + (long long)s * 2 * (g.nbx + 1);
-    const double cc = ordered_sum(pc, g.nbx, red);
+    const double* pc = pchk + (long long)s * 2 * (g.nbc + 1);
+    const double cc = ordered_sum(pc, g.nbc, red);
     double ccp = 0;
-    if (need_prev) ccp = ordered_sum(pc + g.nbx + 1, g.nbx, red);
+    if (need_prev) ccp = ordered_sum(pc + g.nbc + 1, g.nbc, red);
     if (threadIdx.x == 0) {
-        const double eq = cc / pc[g.nbx] * g.invN;
-        const double eqp = need_prev ? (ccp / pc[2 * g.nbx + 1] * g.invN) : c.eq_val;
+        const double eq = cc / pc[g.nbc] * g.invN;
+        const double eqp = need_prev ? (ccp / pc[2 * g.nbc + 1] * g.invN) : c.eq_val;
         c.eqvar = eq;
         c.deq = fabs(eq - eqp);
         c.eq_val = eq;
