@@ -304,28 +304,46 @@ def bench_thermal512(args):
 def bench_j2_192(args):
     """192^3 J2 matrix (E=70 GPa, nu=0.3, sigma_y=100 MPa, H=1 GPa) with 25%
     elastic spheres (E=400 GPa, nu=0.2), uniaxial strain to 2% in 50
-    increments, basic scheme with mixed control, FP64."""
+    increments, basic scheme with mixed control in FP64 (the value), beside
+    the opt-in FP32 phase per increment (FP32 fields, FFTs and return map to
+    10 x tol, then FP64) with its speedup and its difference from FP64."""
     import mat3d_amd as m
     from mat3d_amd import gen
 
     n = 192
     ids = gen.c4_rve(n)
     X0 = np.eye(6)[0]
-    s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
-    s.material(gen.MAT_C4, ids)
-    vi = 0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        s.setLoad(0.02 * X0, X0)
-        info = s.runAnalysis(1e-6, 1e-6, 50)
-        vi += n**3 * info["iters_fp64"]
-    dt = time.perf_counter() - t0
-    P = s.prop_homogenized
-    print(f"[bench] j2_192: {vi / dt:.4e} voxel-it/s, {info['iters_fp64']} iterations over 50 increments, "
-          f"final sigma_11 {P[-1, 1]:.4e}", flush=True)
-    _line(args, "voxel-iters/s, 192^3 J2 50 increments", vi / dt, "voxel-iters/s", dt / args.steps * 1e3, "f64",
-          "192^3 J2 matrix + elastic spheres, uniaxial 2% in 50 increments, basic scheme",
-          {"iterations": info["iters_fp64"], "final_sigma11": P[-1, 1]})
+
+    def run(precision, steps, **opt):
+        s = m.FFT_Analysis3D(n, n, n, "ep", precision=precision)
+        for k, v in opt.items():
+            s.set_option(k, v)
+        s.material(gen.MAT_C4, ids)
+        vi = 0
+        t0 = time.perf_counter()
+        for i in range(steps):
+            s.setLoad(0.02 * X0, X0)
+            info = s.runAnalysis(1e-6, 1e-6, 50)
+            vi += n**3 * (info["iters_fp32"] + info["iters_fp64"])
+        dt = time.perf_counter() - t0
+        eps, sig = s.fields()
+        return vi / dt, dt / steps, info, s.prop_homogenized.copy(), sig
+
+    v64, t64, i64, P64, sig64 = run("fp64", args.steps)
+    print(f"[bench] j2_192 fp64: {v64:.4e} voxel-it/s, {i64['iters_fp64']} iterations over 50 increments, "
+          f"final sigma_11 {P64[-1, 1]:.4e}", flush=True)
+    extra = {"iterations": i64["iters_fp64"], "final_sigma11": P64[-1, 1]}
+    if not args.no_fp64:
+        v32, t32, i32, P32, sig32 = run("fp32_polish", 1, basic_fp32=1)
+        d_prop = float(np.abs(P32 - P64).max() / np.abs(P64).max())
+        d_sig = float(np.abs(sig32 - sig64).max() / np.abs(sig64).max())
+        print(f"[bench] j2_192 fp32 phase (opt-in): {v32:.4e} voxel-it/s, iterations fp32 {i32['iters_fp32']} "
+              f"fp64 {i32['iters_fp64']}; prop diff {d_prop:.2e}, sigma diff {d_sig:.2e}", flush=True)
+        extra["fp32_phase_opt_in"] = {"value": v32, "speedup_vs_fp64": v32 / v64, "iterations_fp32": i32["iters_fp32"],
+                                      "iterations_fp64": i32["iters_fp64"], "prop_rel_diff": d_prop,
+                                      "sigma_rel_diff": d_sig}
+    _line(args, "voxel-iters/s, 192^3 J2 50 increments", v64, "voxel-iters/s", t64 * 1e3, "f64",
+          "192^3 J2 matrix + elastic spheres, uniaxial 2% in 50 increments, basic scheme", extra)
 
 
 def bench_basic64(args):

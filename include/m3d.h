@@ -122,7 +122,13 @@ int m3d_elastic_info(m3d_elastic *h, double *vol, int *strain_base, double *la0,
  * an FP32 correction of one FP64 anchor with FP64 stop-rule sums and checks;
  * 0: plain FP64 polish iterations), "scheme" (0, default: the reference's
  * accelerated "e" loop; 1: the basic Moulinec-Suquet iteration with the load
- * held fixed, for homogenization and the "e" runAnalysis). */
+ * held fixed, for homogenization and the "e" runAnalysis), "basic_fp32" (0,
+ * default: "ep"/"ed" runs iterate in FP64; 1 on an FP32+polish "ep" handle:
+ * each increment iterates in FP32 -- fields, FFTs, return map -- to
+ * polish_factor x tol, then in FP64 under the reference rule; the history
+ * stays FP64.  Plasticity carries the FP32 rounding of the early iterations
+ * into the final fields: about 5e-5 of the largest stress at polish_factor
+ * 10, 1e-5 at 1000). */
 int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 
 /* Kernel timing for measurement tools (no reference counterpart).  Times

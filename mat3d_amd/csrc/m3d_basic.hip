@@ -1,6 +1,6 @@
 // m3d_basic.hip -- compile-time specialised kernels of the basic scheme with
 // mixed control and the J2 / damage updates (runAnalysis "ep" / "ed",
-// src/3d/FFT_analysis3D.cpp:796-853), FP64.  The host driver is BasicRunner
+// src/3d/FFT_analysis3D.cpp:796-853), in FP64 and FP32.  The host driver is BasicRunner
 // (m3d_basic_host.hpp); it calls the launchers at the end of this file for
 // grids whose line lengths have specialised kernels and the runtime-planned
 // kernels of m3d_basic.hpp otherwise.
@@ -27,10 +27,9 @@ struct Comp6 {
 //   then per voxel the constitutive update (elastoPlastic :640-691 or
 //   elastoDamage :619-638) -> sigma; FP64 partial sums of sigma and eps
 //   (updateE :583-594); r2c of the 12 lines -> [sigma^ | eps^]
-template <int BEH, int MODE, int LZ, int TY>
-__global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<double> p)
+template <typename T, int BEH, int MODE, int LZ, int TY>
+__global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 {
-    using T = double;
     constexpr int C = 12;
     constexpr int W = TY * C;
     constexpr int NZ = 2 * LZ;
@@ -199,10 +198,9 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<double> p)
 // sigma^ (convergenceCheck :351-389), Green step t = eps^ - Gamma0:sigma^
 // (multiplyGF :323-338, DC = E) written over the sigma columns, inverse x
 // FFT of those 6 columns per kz.
-template <int L, int TK>
-__global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<double> p)
+template <typename T, int L, int TK>
+__global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
 {
-    using T = double;
     constexpr int C = 12;
     constexpr int W = TK * C;
     constexpr int NT = XNT;
@@ -222,7 +220,7 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<double> p)
     const cx<T>* base = p.S12 + ((long long)y * g.Nzh + kz0) * C;
     const long long xstride = (long long)g.Ny * g.Nzh * C;
     for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
-    fast::rows_to_lds<T, L, W, NT, false>(sl, base, fast::uniform_rows(xstride), nk * C);
+    fast::rows_to_lds<T, L, W, NT, sizeof(T) == 4>(sl, base, fast::uniform_rows(xstride), nk * C);
     __syncthreads();
     fast::fft<T, L, W, NT, -1, 1>(sl, tw);
 
@@ -285,85 +283,105 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<double> p)
 // ---------------------------------------------------------------- launchers
 namespace {
 
-template <int BEH, int MODE, int LZ>
-bool z_at(const Geo& g, const BPtr<double>& p, hipStream_t st)
+template <typename T, int BEH, int MODE, int LZ>
+bool z_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (g.Nz != 2 * LZ) return false;
     constexpr int TY = zlines(LZ, 2 * LZ);
     if (g.TY != TY) throw std::runtime_error("basic z tile mismatch");
-    const size_t sm = ((size_t)(LZ + 1) * TY * 12 + 2 * LZ) * sizeof(cx<double>);
-    hipLaunchKernelGGL((kzb<BEH, MODE, LZ, TY>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
+    const size_t sm = ((size_t)(LZ + 1) * TY * 12 + 2 * LZ) * sizeof(cx<T>);
+    hipLaunchKernelGGL((kzb<T, BEH, MODE, LZ, TY>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
     return true;
 }
 
-template <int BEH, int MODE>
-void z_mode(const Geo& g, const BPtr<double>& p, hipStream_t st)
+template <typename T, int BEH, int MODE>
+void z_mode(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
-    const bool ok = z_at<BEH, MODE, 128>(g, p, st) || z_at<BEH, MODE, 96>(g, p, st) || z_at<BEH, MODE, 64>(g, p, st) ||
-                    z_at<BEH, MODE, 48>(g, p, st) || z_at<BEH, MODE, 32>(g, p, st) || z_at<BEH, MODE, 24>(g, p, st) ||
-                    z_at<BEH, MODE, 16>(g, p, st);
+    const bool ok = z_at<T, BEH, MODE, 128>(g, p, st) || z_at<T, BEH, MODE, 96>(g, p, st) ||
+                    z_at<T, BEH, MODE, 64>(g, p, st) || z_at<T, BEH, MODE, 48>(g, p, st) ||
+                    z_at<T, BEH, MODE, 32>(g, p, st) || z_at<T, BEH, MODE, 24>(g, p, st) ||
+                    z_at<T, BEH, MODE, 16>(g, p, st);
     if (!ok) throw std::runtime_error("no basic z kernel for this grid");
 }
 
-template <typename PH, int DIR, int WHICH, int L>
-bool y_at(const Geo& g, const Ptr<double>& p, hipStream_t st)
+template <typename T, typename PH, int DIR, int WHICH, int L>
+bool y_at(const Geo& g, const Ptr<T>& p, hipStream_t st)
 {
     if (g.Ny != L) return false;
-    constexpr int TK = fast::xytile<double>(L, PH::C);
+    constexpr int TK = fast::xytile<T>(L, PH::C);
     const int nkt = (g.Nzh + TK - 1) / TK;
-    const size_t sm = ((size_t)L * TK * PH::C + L) * sizeof(cx<double>);
-    hipLaunchKernelGGL((fast::kxy<double, PH, 0, DIR, WHICH, L, TK, 256>), dim3(g.Nx * nkt), dim3(256), sm, st, g, p,
-                       (const double*)nullptr);
+    const size_t sm = ((size_t)L * TK * PH::C + L) * sizeof(cx<T>);
+    hipLaunchKernelGGL((fast::kxy<T, PH, 0, DIR, WHICH, L, TK, 256>), dim3(g.Nx * nkt), dim3(256), sm, st, g, p,
+                       (const T*)nullptr);
     return true;
 }
 
-template <typename PH, int DIR, int WHICH>
-void y_run(const Geo& g, const Ptr<double>& p, hipStream_t st)
+template <typename T, typename PH, int DIR, int WHICH>
+void y_run(const Geo& g, const Ptr<T>& p, hipStream_t st)
 {
-    const bool ok = y_at<PH, DIR, WHICH, 256>(g, p, st) || y_at<PH, DIR, WHICH, 192>(g, p, st) ||
-                    y_at<PH, DIR, WHICH, 128>(g, p, st) || y_at<PH, DIR, WHICH, 96>(g, p, st) ||
-                    y_at<PH, DIR, WHICH, 64>(g, p, st) || y_at<PH, DIR, WHICH, 48>(g, p, st) ||
-                    y_at<PH, DIR, WHICH, 32>(g, p, st);
+    const bool ok = y_at<T, PH, DIR, WHICH, 256>(g, p, st) || y_at<T, PH, DIR, WHICH, 192>(g, p, st) ||
+                    y_at<T, PH, DIR, WHICH, 128>(g, p, st) || y_at<T, PH, DIR, WHICH, 96>(g, p, st) ||
+                    y_at<T, PH, DIR, WHICH, 64>(g, p, st) || y_at<T, PH, DIR, WHICH, 48>(g, p, st) ||
+                    y_at<T, PH, DIR, WHICH, 32>(g, p, st);
     if (!ok) throw std::runtime_error("no basic y kernel for this grid");
 }
 
-template <int L>
-bool x_at(const Geo& g, const BPtr<double>& p, hipStream_t st)
+template <typename T, int L>
+bool x_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (g.Nx != L) return false;
     constexpr int TK = xtile(L);
     if (g.TKx != TK) throw std::runtime_error("basic x tile mismatch");
-    const size_t sm = ((size_t)L * TK * 12 + L) * sizeof(cx<double>);
+    const size_t sm = ((size_t)L * TK * 12 + L) * sizeof(cx<T>);
     if (sm > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kxb<L, TK>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)sm);
-    hipLaunchKernelGGL((kxb<L, TK>), dim3(g.nbx), dim3(XNT), sm, st, g, p);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kxb<T, L, TK>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((kxb<T, L, TK>), dim3(g.nbx), dim3(XNT), sm, st, g, p);
     return true;
 }
 
 } // namespace
 
-void launch_z(int beh, int mode, const Geo& g, const BPtr<double>& p, hipStream_t st)
+template <typename T>
+void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (beh == BEH_EP) {
-        if (mode == BMODE_START) z_mode<BEH_EP, BMODE_START>(g, p, st);
-        else z_mode<BEH_EP, BMODE_ITER>(g, p, st);
+        if (mode == BMODE_START) z_mode<T, BEH_EP, BMODE_START>(g, p, st);
+        else z_mode<T, BEH_EP, BMODE_ITER>(g, p, st);
     } else {
-        if (mode == BMODE_START) z_mode<BEH_ED, BMODE_START>(g, p, st);
-        else z_mode<BEH_ED, BMODE_ITER>(g, p, st);
+        if (mode == BMODE_START) z_mode<T, BEH_ED, BMODE_START>(g, p, st);
+        else z_mode<T, BEH_ED, BMODE_ITER>(g, p, st);
     }
 }
 
-void launch_y12(const Geo& g, const Ptr<double>& p, hipStream_t st) { y_run<Comp12, -1, BUF_K>(g, p, st); }
-
-void launch_y6(const Geo& g, const Ptr<double>& p, hipStream_t st) { y_run<Comp6, 1, BUF_MAIN>(g, p, st); }
-
-void launch_x(const Geo& g, const BPtr<double>& p, hipStream_t st)
+template <typename T>
+void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st)
 {
-    const bool ok = x_at<256>(g, p, st) || x_at<192>(g, p, st) || x_at<128>(g, p, st) || x_at<96>(g, p, st) ||
-                    x_at<64>(g, p, st) || x_at<48>(g, p, st) || x_at<32>(g, p, st);
+    y_run<T, Comp12, -1, BUF_K>(g, p, st);
+}
+
+template <typename T>
+void launch_y6(const Geo& g, const Ptr<T>& p, hipStream_t st)
+{
+    y_run<T, Comp6, 1, BUF_MAIN>(g, p, st);
+}
+
+template <typename T>
+void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st)
+{
+    const bool ok = x_at<T, 256>(g, p, st) || x_at<T, 192>(g, p, st) || x_at<T, 128>(g, p, st) ||
+                    x_at<T, 96>(g, p, st) || x_at<T, 64>(g, p, st) || x_at<T, 48>(g, p, st) || x_at<T, 32>(g, p, st);
     if (!ok) throw std::runtime_error("no basic x kernel for this grid");
 }
+
+template void launch_z<double>(int, int, const Geo&, const BPtr<double>&, hipStream_t);
+template void launch_z<float>(int, int, const Geo&, const BPtr<float>&, hipStream_t);
+template void launch_y12<double>(const Geo&, const Ptr<double>&, hipStream_t);
+template void launch_y12<float>(const Geo&, const Ptr<float>&, hipStream_t);
+template void launch_y6<double>(const Geo&, const Ptr<double>&, hipStream_t);
+template void launch_y6<float>(const Geo&, const Ptr<float>&, hipStream_t);
+template void launch_x<double>(const Geo&, const BPtr<double>&, hipStream_t);
+template void launch_x<float>(const Geo&, const BPtr<float>&, hipStream_t);
 
 } // namespace bfast
 } // namespace m3d

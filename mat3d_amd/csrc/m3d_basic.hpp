@@ -26,7 +26,7 @@
 namespace m3d {
 
 enum { BEH_EP = 1, BEH_ED = 2 };
-enum { BMODE_START = 0, BMODE_ITER = 1 };
+enum { BMODE_START = 0, BMODE_ITER = 1, BMODE_RESTART = 2 };
 constexpr int NBSUM = 12; // sums per solve in the z pass: sigma_c, eps_c
 
 template <typename T>
@@ -93,13 +93,25 @@ __device__ __forceinline__ bool j2_update(const double* m, const T* eps, const T
     if (phi > T(0)) {
         T x = 0, x_o = 0, res;
         bool ok = false;
-        for (int iter = 0; iter < 200; ++iter) {
-            x = x_o - (seff - T(3) * mu * x_o - Sy - H * pow(ep_eff + x_o, n)) /
-                          (-T(3) * mu - H * n * pow(ep_eff + x_o, n - T(1)));
-            x_o = x;
-            res = seff - T(3) * mu * x - Sy - H * pow(ep_eff + x, n);
-            if (isnan(x)) break;
-            if (fabs(res) < T(1e-5)) { ok = true; break; }
+        if (sizeof(T) == 4 && n == T(1)) {
+            // FP32 phase, linear hardening: the root of
+            // seff - 3 mu x - Sy - H (ep + x) = 0 (the reference's first Newton
+            // step from x = 0 is this root)
+            x = phi / (T(3) * mu + H);
+            ok = true;
+        } else {
+            // FP32: |res| < 1e-5 is below the resolution of stresses in FP32,
+            // so the FP32 phase stops at a residual relative to the yield
+            // stress; the FP64 polish applies the reference's absolute rule
+            const T rtol = sizeof(T) == 4 ? T(1e-6) * (Sy + seff) : T(1e-5);
+            for (int iter = 0; iter < 200; ++iter) {
+                x = x_o - (seff - T(3) * mu * x_o - Sy - H * pow(ep_eff + x_o, n)) /
+                              (-T(3) * mu - H * n * pow(ep_eff + x_o, n - T(1)));
+                x_o = x;
+                res = seff - T(3) * mu * x - Sy - H * pow(ep_eff + x, n);
+                if (isnan(x)) break;
+                if (fabs(res) < rtol) { ok = true; break; }
+            }
         }
         if (!ok) return false;
         dep = x;
@@ -451,6 +463,9 @@ __device__ __forceinline__ void update_E(Ctl& c, const double* sbar, const doubl
 //   sum_field_old = sum(sigma^0); E <- updateE(sigma^0, eps^0); itNo = 0
 // MODE_ITER : itNo += 1, diff_sum_field with the E that produced this
 //   iterate (:399), sum_field_old for the next check, E <- updateE
+// MODE_RESTART: the iterate itNo recomputed in FP64 from its FP32 strain
+//   (end of an FP32 phase): sum_field_old and E as in MODE_ITER, itNo kept;
+//   the check after it records eqVar without a stop decision
 template <int MODE>
 __global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const double* psum)
 {
@@ -475,6 +490,8 @@ __global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const doubl
             for (int q = 0; q < 6; ++q) es += c.E[q];
             c.dsum = fabs((S - c.sum_prev) / (c.nc * es));
             c.first = 0;
+        } else if (MODE == BMODE_RESTART) {
+            c.first = 2;
         } else {
             c.it = 0;
             c.first = 1;
@@ -497,8 +514,15 @@ __global__ void __launch_bounds__(FTHREADS) k_bfin2(Geo g, Ctl* ctl, const doubl
     const double* pc = pchk + (long long)s * (g.nbx + 1);
     const double cc = ordered_sum(pc, g.nbx, red);
     if (threadIdx.x == 0) {
-        if (c.first) return; // the stage after the increment start checks nothing
+        if (c.first == 1) return; // the stage after the increment start checks nothing
         const double eq = cc / pc[g.nbx] * g.invN;
+        if (c.first == 2) {
+            // restart stage: eqVar of the recomputed iterate, no stop decision
+            c.eq_val = eq;
+            c.eqvar = eq;
+            c.first = 0;
+            return;
+        }
         c.deq = fabs(eq - c.eq_val);
         c.eq_val = eq;
         c.eqvar = eq;
@@ -552,6 +576,33 @@ __global__ void k_bupdate(Geo g, BPtr<T> p, int nsolve)
             }
         }
     }
+}
+
+// copy fields between the FP64 and FP32 states of a solve; what: 1 eps,
+// 2 sig, 4 sig_old, 8 eps_old, 16 ep|d, 32 phi, dep and seff, 64 the Green
+// output spectrum S6 (6 x H complex)
+enum { BC_EPS = 1, BC_SIG = 2, BC_SIGOLD = 4, BC_EPSOLD = 8, BC_EP = 16, BC_PHI = 32, BC_S6 = 64 };
+template <typename TS, typename TD>
+__global__ void k_bcopy(Geo g, BPtr<TS> a, BPtr<TD> b, int what)
+{
+    const long long n6 = 6 * g.N, n1 = g.N, h6 = 6 * g.H;
+    const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, st = (long long)gridDim.x * blockDim.x;
+    for (long long i = i0; i < n6; i += st) {
+        if (what & BC_EPS) b.eps[i] = TD(a.eps[i]);
+        if (what & BC_SIG) b.sig[i] = TD(a.sig[i]);
+        if (what & BC_SIGOLD) b.sig_old[i] = TD(a.sig_old[i]);
+        if (what & BC_EPSOLD) b.eps_old[i] = TD(a.eps_old[i]);
+    }
+    for (long long i = i0; i < n1; i += st) {
+        if (what & BC_EP) b.ep[i] = TD(a.ep[i]);
+        if (what & BC_PHI) {
+            b.phi[i] = TD(a.phi[i]);
+            b.dep[i] = TD(a.dep[i]);
+            b.seff[i] = TD(a.seff[i]);
+        }
+    }
+    if (what & BC_S6)
+        for (long long i = i0; i < h6; i += st) b.S6[i] = cx<TD>{TD(a.S6[i].re), TD(a.S6[i].im)};
 }
 
 template <typename T>

@@ -43,10 +43,15 @@ inline bool supported(int Nx, int Ny, int Nz)
 // host launchers (m3d_basic.hip); g carries Nx..Nzh, N, H, invN, nbz, nbx
 // of the tiles above, TY, TKx; the y launch takes the 12- or 6-component
 // view of the spectra in p (S: 6-component Green output, K: 12-component)
-void launch_z(int beh, int mode, const Geo& g, const BPtr<double>& p, hipStream_t st);
-void launch_y12(const Geo& g, const Ptr<double>& p, hipStream_t st);
-void launch_x(const Geo& g, const BPtr<double>& p, hipStream_t st);
-void launch_y6(const Geo& g, const Ptr<double>& p, hipStream_t st);
+// (T = double or float)
+template <typename T>
+void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st);
+template <typename T>
+void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st);
+template <typename T>
+void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st);
+template <typename T>
+void launch_y6(const Geo& g, const Ptr<T>& p, hipStream_t st);
 
 } // namespace bfast
 } // namespace m3d

@@ -7,7 +7,12 @@
 namespace {
 
 // ------------------------------------------------------ basic scheme runner
-// runAnalysis "ep" / "ed" (FFT_analysis3D.cpp:796-853) in FP64 on one solve.
+// runAnalysis "ep" / "ed" (FFT_analysis3D.cpp:796-853) on one solve.  FP64
+// handles, and grids without specialised kernels, iterate in FP64.  On an
+// FP32+polish handle each increment iterates in FP32 (fields, FFTs, return
+// map) until the stop rule holds at polish_factor x tol, then continues in
+// FP64 from the promoted Green output under the reference's rule; the
+// history (sigma_old, eps_old, ep) is kept and updated in FP64.
 struct BasicRunner {
     Engine<Elastic>& e;
     Geo g12{}, g6{};
@@ -15,10 +20,15 @@ struct BasicRunner {
     Geo gf6{};         // the same with the row strides of the 6-component spectrum
     bool fast = false; // grid has specialised basic kernels (m3d_basic.hip)
     int beh;
+    bool two = false;       // FP32 phase + FP64 polish per increment
     double* f6 = nullptr;   // eps, sig, sig_old, eps_old (4 x 6N)
     double* f1 = nullptr;   // ep|d, phi, dep, seff (4 x N)
     unsigned char* S6 = nullptr;
     unsigned char* S12 = nullptr;
+    float* f6s = nullptr;   // FP32 phase: the same fields in FP32
+    float* f1s = nullptr;
+    unsigned char* S6s = nullptr;
+    unsigned char* S12s = nullptr;
     double* psum = nullptr;
     double* pchk = nullptr;
     double* bs = nullptr;
@@ -30,7 +40,7 @@ struct BasicRunner {
     int exit_inc = -1; // increment of the "ed" softening exit, -1 if none
     m3d_run_info info{};
 
-    BasicRunner(Engine<Elastic>& eng, int b) : e(eng), beh(b)
+    BasicRunner(Engine<Elastic>& eng, int b, bool fp32) : e(eng), beh(b)
     {
         g12 = make_geo<double>(e.Nx, e.Ny, e.Nz, 12);
         g6 = make_geo<double>(e.Nx, e.Ny, e.Nz, 6);
@@ -72,6 +82,13 @@ struct BasicRunner {
             e.alloc(&bs, sizeof(double) * 14 * nblk);
             e.alloc(&mat, sizeof(double) * 256 * 8);
             e.alloc(&ctl, sizeof(Ctl));
+            two = fp32 && fast;
+            if (two) {
+                e.alloc(&f6s, sizeof(float) * 24 * N);
+                e.alloc(&f1s, sizeof(float) * 4 * N);
+                e.alloc(&S6s, sizeof(cx<float>) * 12 * H);
+                e.alloc(&S12s, sizeof(cx<float>) * 12 * H);
+            }
         } catch (...) {
             release();
             throw;
@@ -80,55 +97,65 @@ struct BasicRunner {
     void release()
     {
         for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)psum, (void*)pchk, (void*)bs, (void*)mat,
-                        (void*)ctl})
+                        (void*)ctl, (void*)f6s, (void*)f1s, (void*)S6s, (void*)S12s})
             if (q) (void)hipFree(q);
         f6 = f1 = psum = pchk = bs = mat = nullptr;
         S6 = S12 = nullptr;
+        f6s = f1s = nullptr;
+        S6s = S12s = nullptr;
         ctl = nullptr;
     }
     ~BasicRunner() { release(); }
 
-    BPtr<double> bptr()
+    template <typename T>
+    BPtr<T> bptr_t()
     {
-        BPtr<double> p;
+        BPtr<T> p;
         std::memset(&p, 0, sizeof p);
         const long long N = e.N;
-        p.eps = f6;
-        p.sig = f6 + 6 * N;
-        p.sig_old = f6 + 12 * N;
-        p.eps_old = f6 + 18 * N;
-        p.ep = f1;
-        p.phi = f1 + N;
-        p.dep = f1 + 2 * N;
-        p.seff = f1 + 3 * N;
+        T* a6 = sizeof(T) == 8 ? (T*)f6 : (T*)f6s;
+        T* a1 = sizeof(T) == 8 ? (T*)f1 : (T*)f1s;
+        p.eps = a6;
+        p.sig = a6 + 6 * N;
+        p.sig_old = a6 + 12 * N;
+        p.eps_old = a6 + 18 * N;
+        p.ep = a1;
+        p.phi = a1 + N;
+        p.dep = a1 + 2 * N;
+        p.seff = a1 + 3 * N;
         p.sU = 6 * N;
         p.sN = N;
-        p.S6 = reinterpret_cast<cx<double>*>(S6);
-        p.S12 = reinterpret_cast<cx<double>*>(S12);
+        p.S6 = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
+        p.S12 = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S12 : S12s);
         p.sS6 = p.sS12 = 12 * e.H;
         p.ph = e.ph;
         p.mat = mat;
-        p.twz = e.tw64[0];
-        p.twy = e.tw64[1];
-        p.twx = e.tw64[2];
+        const cx<T>* const* tw = e.template twiddles_of<T>();
+        p.twz = tw[0];
+        p.twy = tw[1];
+        p.twx = tw[2];
         p.psum = psum;
         p.pchk = pchk;
         p.ctl = ctl;
         return p;
     }
-    Ptr<double> yptr()
+    template <typename T>
+    Ptr<T> yptr_t()
     {
-        Ptr<double> p;
+        Ptr<T> p;
         std::memset(&p, 0, sizeof p);
-        p.S = reinterpret_cast<cx<double>*>(S6);
-        p.K = reinterpret_cast<cx<double>*>(S12);
+        p.S = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
+        p.K = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S12 : S12s);
         p.sS = 12 * e.H;
-        p.twz = e.tw64[0];
-        p.twy = e.tw64[1];
-        p.twx = e.tw64[2];
+        const cx<T>* const* tw = e.template twiddles_of<T>();
+        p.twz = tw[0];
+        p.twy = tw[1];
+        p.twx = tw[2];
         p.ctl = ctl;
         return p;
     }
+    BPtr<double> bptr() { return bptr_t<double>(); }
+    Ptr<double> yptr() { return yptr_t<double>(); }
 
     template <bool GEN>
     void stage(int mode)
@@ -159,19 +186,23 @@ struct BasicRunner {
         hipLaunchKernelGGL((k_ypass<double, 6, 1, BUF_MAIN, GEN>), dim3(g6.Nx * g6.nky), dim3(XYTHREADS), ysm6, st, g6, yp);
     }
 
-    // one stage with the specialised kernels (same sequence as stage())
+    // one stage with the specialised kernels (same sequence as stage()); the
+    // restart stage is a start stage (strain from HBM) that keeps itNo
+    template <typename T = double>
     void stage_fast(int mode)
     {
         hipStream_t st = e.st;
-        BPtr<double> bp = bptr();
-        Ptr<double> yp = yptr();
-        bfast::launch_z(beh, mode, gf, bp, st);
+        BPtr<T> bp = bptr_t<T>();
+        Ptr<T> yp = yptr_t<T>();
+        bfast::launch_z<T>(beh, mode == BMODE_ITER ? BMODE_ITER : BMODE_START, gf, bp, st);
         if (mode == BMODE_START) hipLaunchKernelGGL((k_bfin1<BMODE_START>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
+        else if (mode == BMODE_RESTART)
+            hipLaunchKernelGGL((k_bfin1<BMODE_RESTART>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
         else hipLaunchKernelGGL((k_bfin1<BMODE_ITER>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
-        bfast::launch_y12(gf, yp, st);
-        bfast::launch_x(gf, bp, st);
+        bfast::launch_y12<T>(gf, yp, st);
+        bfast::launch_x<T>(gf, bp, st);
         hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, pchk);
-        bfast::launch_y6(gf6, yp, st);
+        bfast::launch_y6<T>(gf6, yp, st);
     }
 
     void run_stage(int mode)
@@ -213,29 +244,43 @@ struct BasicRunner {
         prop.assign((size_t)9 * nInc, 0.0);
         rows = nInc;
         exit_inc = -1;
-        int nx = 0, total_it = 0;
+        int nx = 0, total_it = 0, total32 = 0;
         std::vector<double> part((size_t)14 * nblk);
         const int ck = 4;
-        // one captured graph of `ck` iterations, replayed in every increment
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < ck; ++k) run_stage(BMODE_ITER);
-        HCHK(hipStreamEndCapture(st, &graph));
-        hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ge != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            throw HipError{ge, "hipGraphInstantiate"};
-        }
+        // captured graphs of `ck` iterations (FP64, and FP32 for the FP32
+        // phase), replayed in every increment
         struct GraphGuard {
-            hipGraph_t g;
-            hipGraphExec_t x;
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
             ~GraphGuard()
             {
-                (void)hipGraphExecDestroy(x);
-                (void)hipGraphDestroy(g);
+                if (x) (void)hipGraphExecDestroy(x);
+                if (g) (void)hipGraphDestroy(g);
             }
-        } guard_graph{graph, exec};
+        } g64, g32;
+        auto capture = [&](GraphGuard& gg, bool fp32) {
+            HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < ck; ++k) {
+                if (fp32) stage_fast<float>(BMODE_ITER);
+                else run_stage(BMODE_ITER);
+            }
+            HCHK(hipStreamEndCapture(st, &gg.g));
+            hipError_t ge = hipGraphInstantiate(&gg.x, gg.g, nullptr, nullptr, 0);
+            if (ge != hipSuccess) throw HipError{ge, "hipGraphInstantiate"};
+        };
+        capture(g64, false);
+        if (two) capture(g32, true);
+        auto replay = [&](hipGraphExec_t x) {
+            for (int guard = 0; guard < 100000; ++guard) {
+                HCHK(hipGraphLaunch(x, st));
+                HCHK(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
+                HCHK(hipStreamSynchronize(st));
+                if (c.done) break;
+            }
+        };
+        BPtr<float> bp32{};
+        if (two) bp32 = bptr_t<float>();
+        const int nb = 1024;
         HCHK(hipEventRecord(e.ev0, st));
         for (int inc = 0; inc < nInc; ++inc) {
             // host copy of the device state, then reset the per-increment fields
@@ -245,14 +290,53 @@ struct BasicRunner {
             c.done = 0;
             c.it = 0;
             c.err = 0;
-            HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
-            run_stage(BMODE_START);
-            HCHK(hipGetLastError());
-            for (int guard = 0; guard < 100000; ++guard) {
-                HCHK(hipGraphLaunch(exec, st));
-                HCHK(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
-                HCHK(hipStreamSynchronize(st));
-                if (c.done) break;
+            bool fp64_run = true;
+            if (two) {
+                // FP32 phase from the increment's FP64 start state
+                c.tol_f = std::max(tol_f * e.polish_factor, e.fp32_floor);
+                c.tol_eq = std::max(tol_eq * e.polish_factor, e.fp32_floor);
+                HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL((k_bcopy<double, float>), dim3(nb), dim3(256), 0, st, g12, bp, bp32,
+                                   BC_EPS | BC_SIGOLD | BC_EPSOLD | BC_EP);
+                stage_fast<float>(BMODE_START);
+                HCHK(hipGetLastError());
+                replay(g32.x);
+                if (c.done == 2) {
+                    // iteration cap reached in FP32: that state is the result
+                    hipLaunchKernelGGL((k_bcopy<float, double>), dim3(nb), dim3(256), 0, st, g12, bp32, bp,
+                                       BC_EPS | BC_SIG | BC_EP | BC_PHI);
+                    total32 += c.it;
+                    fp64_run = false;
+                } else if (c.done == 1) {
+                    // the FP32 rule holds at iterate itNo: recompute that iterate
+                    // in FP64 from its strain (and the damage of its constitutive
+                    // call), then iterate in FP64 under the reference rule
+                    hipLaunchKernelGGL((k_bcopy<float, double>), dim3(nb), dim3(256), 0, st, g12, bp32, bp,
+                                       BC_EPS | (beh == BEH_ED ? BC_EP : 0));
+                    total32 += c.it;
+                    c.done = 0;
+                    c.tol_f = tol_f;
+                    c.tol_eq = tol_eq;
+                    HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+                    stage_fast<double>(BMODE_RESTART);
+                    HCHK(hipGetLastError());
+                    replay(g64.x);
+                    fp64_run = false;
+                } else {
+                    // the FP32 return map failed: redo the increment in FP64 (its
+                    // start state is untouched)
+                    c.done = 0;
+                    c.it = 0;
+                    c.err = 0;
+                }
+            }
+            if (fp64_run) {
+                c.tol_f = tol_f;
+                c.tol_eq = tol_eq;
+                HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+                run_stage(BMODE_START);
+                HCHK(hipGetLastError());
+                replay(g64.x);
             }
             if (c.done == 3)
                 throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};
@@ -302,8 +386,8 @@ struct BasicRunner {
         float ms = 0;
         HCHK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
         info.time_ms = ms;
-        info.iters_fp64 = total_it;
-        info.iters_fp32 = 0;
+        info.iters_fp64 = total_it - total32;
+        info.iters_fp32 = total32;
     }
 
     void fields(double* eps, double* sig)

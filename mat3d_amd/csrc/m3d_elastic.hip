@@ -12,6 +12,7 @@ struct m3d_elastic {
     std::string behav;
     bool have_mat = false, have_load = false, ran = false;
     int scheme = 0; // 0 accelerated (reference "e" loop), 1 basic with the load held fixed
+    int basic_fp32 = 0; // "ep" on an FP32+polish handle: FP32 phase per increment (opt-in)
     ElasticProps P;
     double E[6] = {0, 0, 0, 0, 0, 0}, X0[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -131,7 +132,13 @@ int m3d_elastic_set_option(m3d_elastic* h, const char* key, double value)
         else if (k == "chunk" && value >= 0) h->eng->chunk = (int)value;
         else if (k == "refine" && (value == 0 || value == 1)) h->eng->refine = (int)value;
         else if (k == "autotune" && (value == 0 || value == 1)) h->eng->autotune = (int)value;
-        else if (k == "scheme" && (value == 0 || value == 1)) {
+        else if (k == "basic_fp32" && (value == 0 || value == 1)) {
+            h->basic_fp32 = (int)value;
+            delete h->basic; // buffers differ; results of an earlier run go with it
+            h->basic = nullptr;
+            if (h->last_basic) h->ran = false;
+            h->last_basic = false;
+        } else if (k == "scheme" && (value == 0 || value == 1)) {
             h->scheme = (int)value;
             if (h->have_mat) {
                 std::vector<double> coef((size_t)256 * NCOEF);
@@ -210,7 +217,9 @@ int m3d_elastic_run(m3d_elastic* h, double tol_field, double tol_eq, int nInc, m
         if (h->behav != "e") {
             // basic scheme with mixed control, J2 ("ep") or damage ("ed"), :796-853
             if (nInc < 1) throw ArgError{M3D_ERR_ARG, "run: nInc must be >= 1"};
-            if (!h->basic) h->basic = new BasicRunner(*h->eng, h->behav == "ep" ? BEH_EP : BEH_ED);
+            if (!h->basic)
+                h->basic = new BasicRunner(*h->eng, h->behav == "ep" ? BEH_EP : BEH_ED,
+                                           h->eng->precision == M3D_FP32_POLISH && h->basic_fp32 && h->behav == "ep");
             const int p1 = h->P.nph > 1 ? 1 : 0;
             const double nc = (double)h->eng->N * (0.5 * h->P.Cm[0] + 0.5 * h->P.Cm[36 * p1]);
             h->basic->run(h->P.mtab, h->P.la0, h->P.mu0, nc, h->E, h->X0, tol_field, tol_eq, nInc, h->eng->cap);

@@ -312,6 +312,13 @@ struct Engine {
     double* prm64 = nullptr;
     cx<float>* tw32[3] = {nullptr, nullptr, nullptr};
     cx<double>* tw64[3] = {nullptr, nullptr, nullptr};
+    // twiddle tables {z, y, x} of one precision
+    template <typename T>
+    const cx<T>* const* twiddles_of() const
+    {
+        if constexpr (sizeof(T) == 8) return tw64;
+        else return tw32;
+    }
     double* fbuf = nullptr; // field output staging (2*C*N doubles)
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -18,14 +18,14 @@ MAT_ED = np.array([[400e9, 0.20, np.inf, 0, 1, 1e-3, 2e3],
                    [70e9, 0.30, np.inf, 0, 1, 1e-3, 8e2]])
 
 
-def run_both(shape, behav, mat, ids, E, X0, nInc, tol=1e-6):
+def run_both(shape, behav, mat, ids, E, X0, nInc, tol=1e-6, precision="fp64"):
     o = oracle.Elastic(*shape, behav)
     o.material(mat, ids)
     o.set_load(E, X0)
     it_o = o.run(tol, tol, nInc)
     P_o = o.prop_homogenized()
     eps_o, sig_o = o.fields()
-    s = m.FFT_Analysis3D(*shape, behav, precision="fp64")
+    s = m.FFT_Analysis3D(*shape, behav, precision=precision)
     s.material(mat, ids)
     s.setLoad(E, X0)
     info = s.runAnalysis(tol, tol, nInc)
@@ -68,6 +68,58 @@ def test_specialised_kernels_match_oracle(shape, behav):
     # differences grow to ~1e-9 in the fields
     ftol = 1e-9 if behav == "ep" else 1e-7
     assert rel(s, s_o) < ftol and rel(e, e_o) < ftol
+
+
+@pytest.mark.parametrize("shape,pf,ptol,ftol", [((32, 32, 32), 1000, 2e-7, 1e-5), ((32, 32, 32), 10, 3e-6, 1e-4),
+                                                 ((32, 64, 48), 10, 3e-6, 1e-4)])
+def test_j2_fp32_phase_against_oracle(shape, pf, ptol, ftol):
+    """Opt-in FP32 phase of "ep" (option basic_fp32 on an FP32+polish handle):
+    each increment iterates in FP32 (fields, FFTs, return map) to pf x tol,
+    then in FP64 under the reference rule from the FP64 recomputation of the
+    switch iterate; the history stays FP64.  The early FP32 rounding stays in
+    the plastic state, so the fields meet 1e-5 of the FP64 oracle only with
+    a small FP32 share (pf 1000); at pf 10 they are within 1e-4.  The FP32
+    and FP64 iterations add up to the oracle's count."""
+    n = max(shape)
+    ids = gen.spheres(n, 4, 0.25, seed=4)[: shape[0], : shape[1], : shape[2]].copy()
+    X0 = np.eye(6)[0]
+    o = oracle.Elastic(*shape, "ep")
+    o.material(gen.MAT_C4, ids)
+    o.set_load(0.02 * X0, X0)
+    it_o = o.run(1e-6, 1e-6, 4)
+    P_o = o.prop_homogenized()
+    e_o, s_o = o.fields()
+    s = m.FFT_Analysis3D(*shape, "ep", precision="fp32_polish")
+    s.set_option("basic_fp32", 1)
+    s.set_option("polish_factor", pf)
+    s.material(gen.MAT_C4, ids)
+    s.setLoad(0.02 * X0, X0)
+    info = s.runAnalysis(1e-6, 1e-6, 4)
+    e, sg = s.fields()
+    P = s.prop_homogenized
+    print(f"[j2 fp32 phase {shape} pf {pf}] iters oracle {it_o} gpu fp32 {info['iters_fp32']} fp64 {info['iters_fp64']} "
+          f"prop {rel(P, P_o):.2e} sigma {rel(sg, s_o):.2e} eps {rel(e, e_o):.2e}")
+    assert info["iters_fp32"] > 0
+    assert abs(info["iters_fp32"] + info["iters_fp64"] - it_o) <= 4
+    assert P.shape == P_o.shape
+    assert rel(P, P_o) < ptol
+    assert rel(sg, s_o) < ftol and rel(e, e_o) < ftol
+
+
+def test_basic_fp32_handle_defaults_to_fp64():
+    """Without the opt-in an FP32+polish handle runs "ep"/"ed" in FP64: the
+    same iterations and results as an FP64 handle."""
+    n = 32
+    ids = gen.spheres(n, 4, 0.25, seed=4)
+    X0 = np.eye(6)[1]
+    out = []
+    for prec in ("fp64", "fp32_polish"):
+        s = m.FFT_Analysis3D(n, n, n, "ed", precision=prec)
+        s.material(MAT_ED, ids)
+        s.setLoad(0.01 * X0, X0)
+        out.append((s.runAnalysis(1e-6, 1e-6, 2), s.prop_homogenized.copy()))
+    assert out[1][0]["iters_fp32"] == 0 and out[1][0]["iters_fp64"] == out[0][0]["iters_fp64"]
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 def test_damage_matches_oracle():
