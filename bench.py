@@ -37,12 +37,12 @@ METRIC = "voxel-iters/s at 256³ (FP32+FP64 polish vs FP64 GPU), C_eff rel err"
 # passes, each scale factor taken from a copy kernel of known size
 # (profiles/elastic256_fibres_hbm_traffic.md)
 MEASURED_TRAFFIC_256 = {
-    "zpass_update": 9.803e9,
-    "ypass_fwd": 5.087e9,
-    "xpass_green": 5.088e9,
-    "ypass_inv": 5.069e9,
+    "zpass_update": 9.805e9,
+    "ypass_fwd": 5.078e9,
+    "xpass_green": 5.163e9,
+    "ypass_inv": 5.099e9,
     "zpass_check": 4.952e9,
-    "xpass_check": 2.441e9,
+    "xpass_check": 2.436e9,
 }
 
 
@@ -177,6 +177,8 @@ def main():
     it_gbs = B * N * geom["solves"] / (it_ms * 1e-3) / 1e9
     print(f"[bench] kernels: {json.dumps(kernels)}", flush=True)
     s.close()
+    copy_gbs = copy_peak()
+    print(f"[bench] device copy (read + write): {copy_gbs:.1f} GB/s", flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -210,7 +212,8 @@ def main():
         "c_eff_rel_err_vs_fp64_gpu": c_err,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kernels[dom]["GB/s"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac"],
-                     "traffic": (MEASURED_TRAFFIC_256.get(dom) if (n == 256 and geom["solves"] == 6) else None)},
+                     "traffic": (MEASURED_TRAFFIC_256.get(dom) if (n == 256 and geom["solves"] == 6) else None),
+                     "copy_measured": round(copy_gbs, 1), "frac_of_copy": round(kernels[dom]["GB/s"] / copy_gbs, 4)},
         "roofline_iteration": {"bytes_per_voxel_iter": round(B, 2), "ms_per_iter_batch": round(it_ms, 4),
                                "achieved": round(it_gbs, 1), "frac": round(it_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
@@ -220,18 +223,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def copy_calibration(gib):
-    """A fill (writes B bytes) and three copies (read B, write B each) of
-    B = gib GiB, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
-    import torch
+def copy_peak(gib=2.0, reps=5):
+    """Read+write rate of the runtime's device copy kernel over gib GiB: the
+    practical streaming ceiling beside the 8 TB/s vendor peak."""
+    import mat3d_amd as m
 
-    n = int(gib * 2**30) // 4
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"[bench] copy calibration: fill + 3 copies of {4 * n} bytes", flush=True)
+    ms = m.device_copy(gib, reps)
+    return 2.0 * gib * 2**30 / (ms * 1e-3) / 1e9
+
+
+def copy_calibration(gib):
+    """A fill (writes B bytes) and four copies (read B, write B each) of
+    B = gib GiB, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import mat3d_amd as m
+
+    ms = m.device_copy(gib, 3)
+    print(f"[bench] copy calibration: fill + 4 copies of {int(gib * 2**30)} bytes, {ms:.3f} ms per copy", flush=True)
 
 
 def _line(args, metric, value, unit, ms, dtype, workload, extra):

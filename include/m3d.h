@@ -149,6 +149,13 @@ int m3d_elastic_get_tuning(m3d_elastic *h, int *xg_threads, int *xg_wide, int *y
 
 const char *m3d_elastic_last_error(m3d_elastic *h);
 
+/* Device-to-device copy rate for measurement tools (no reference
+ * counterpart): one fill and one untimed copy of gib GiB, then `reps` timed
+ * copies with the runtime's copy kernel (HIP events on the null stream);
+ * ms_per_copy = average ms per copy (each reads and writes gib GiB).  Also
+ * the known-size traffic that calibrates the HBM byte counters. */
+int m3d_device_copy(double gib, int reps, double *ms_per_copy);
+
 /* ---------------------------------------------------------------- thermal */
 
 /* FFT_analysis3d_thermal(int Nx, int Ny, int Nz)

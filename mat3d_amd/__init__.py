@@ -36,7 +36,7 @@ EXPORTS = [
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
     "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop", "m3d_elastic_get_state",
-    "m3d_elastic_get_tuning",
+    "m3d_elastic_get_tuning", "m3d_device_copy",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
@@ -91,6 +91,7 @@ def lib():
             "m3d_elastic_get_prop": (ip, [vp, dp, ipp]),
             "m3d_elastic_get_state": (ip, [vp, dp, dp]),
             "m3d_elastic_get_tuning": (ip, [vp, ipp, ipp, ipp]),
+            "m3d_device_copy": (ip, [ct.c_double, ip, dp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -128,6 +129,16 @@ def _i32p(a):
 
 def _colmajor(m):
     return np.ascontiguousarray(np.asarray(m, dtype=np.float64).T).reshape(-1)
+
+
+def device_copy(gib=2.0, reps=5):
+    """Average ms of one device-to-device copy of gib GiB (the runtime's copy
+    kernel) after a fill and an untimed copy; for measurement tools."""
+    ms = ct.c_double(0)
+    rc = lib().m3d_device_copy(float(gib), int(reps), ct.byref(ms))
+    if rc != 0:
+        raise M3DError(rc, "m3d_device_copy failed")
+    return ms.value
 
 
 class FFT_Analysis3D:

@@ -309,6 +309,34 @@ int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, in
     });
 }
 
+// device copy rate for measurement tools (no reference counterpart)
+int m3d_device_copy(double gib, int reps, double* ms_per_copy)
+{
+    if (!(gib > 0) || reps < 1 || !ms_per_copy) return M3D_ERR_ARG;
+    const size_t bytes = (size_t)(gib * 1073741824.0) / 256 * 256;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = M3D_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = M3D_ERR_OOM;
+    if (rc == M3D_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = M3D_ERR_HIP;
+    if (rc == M3D_OK) {
+        float ms = 0;
+        bool ok = hipMemset(a, 1, bytes) == hipSuccess && hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice) == hipSuccess &&
+                  hipEventRecord(e0, nullptr) == hipSuccess;
+        for (int r = 0; ok && r < reps; ++r) ok = hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr) == hipSuccess;
+        ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok) *ms_per_copy = (double)ms / reps;
+        else rc = M3D_ERR_HIP;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    (void)hipGetLastError();
+    return rc;
+}
+
 // ------------------------------------------------------------------ batch
 int m3d_batch_create(int precision, m3d_batch** out)
 {
