@@ -239,6 +239,39 @@ const char *m3d_batch_last_error(m3d_batch *b);
 /* library version string */
 const char *m3d_version(void);
 
+/* ---------------------------------------------------------------------
+ * Phase-retrieval reconstruction (src/rec/reconstruction.cpp), FP64.
+ * ------------------------------------------------------------------- */
+typedef struct m3d_rec m3d_rec;
+
+/* recPlan(double *TPCF, int Nx1, int Ny1, int Nz1) -- include/reconstruction.hpp:55,
+ * src/rec/reconstruction.cpp:211-243.  tpcf: Nx*Ny*Nz values, index
+ * (x*Ny + y)*Nz + z (the reference's k + j*Nz + i*Nz*Ny); normC =
+ * sqrt(|tpcf| * Nx*Ny*Nz).  Each length must be 32, 48, 64, 96, 128, 192 or
+ * 256 (M3D_ERR_UNSUPPORTED otherwise). */
+int m3d_rec_create(const double *tpcf, int nx, int ny, int nz, m3d_rec **out);
+int m3d_rec_destroy(m3d_rec *r);
+const char *m3d_rec_last_error(m3d_rec *r);
+
+/* Start field in place of the random one (no reference counterpart; for
+ * tests and warm starts): Nx*Ny*Nz real values. */
+int m3d_rec_set_initial(m3d_rec *r, const double *m0);
+
+/* runRec(int n, double b) -- include/reconstruction.hpp:66,
+ * src/rec/reconstruction.cpp:285-406.  The reference draws its random
+ * phases with Eigen's setRandom; here value i of stream k is
+ * splitmix64(seed + (k << 40) + i) mapped to [-1, 1): stream 0 starts the
+ * field ((u+1)/2), stream k is the k-th re-randomisation (every 150
+ * iterations).  iterations = entries of trackrealerrorA written; status 1 =
+ * error below 0.1, 2 = iteration number exceeded; search_steps = steps of
+ * the final volume-fraction search. */
+int m3d_rec_run(m3d_rec *r, int n, double beta, uint64_t seed, int *iterations, int *status, int *search_steps);
+
+/* public mTilda (interleaved re, im; Nx*Ny*Nz complex) and trackrealerrorA
+ * (up to maxn values; nerr = how many the last run wrote) --
+ * include/reconstruction.hpp:73-74. */
+int m3d_rec_result(m3d_rec *r, double *mtilda_ri, double *err, int maxn, int *nerr);
+
 #ifdef __cplusplus
 }
 #endif

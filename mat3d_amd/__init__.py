@@ -42,6 +42,7 @@ EXPORTS = [
     "m3d_thermal_set_option", "m3d_thermal_last_error",
     "m3d_batch_create", "m3d_batch_destroy", "m3d_batch_add", "m3d_batch_homogenization", "m3d_batch_results",
     "m3d_batch_stats", "m3d_batch_set_option", "m3d_batch_last_error",
+    "m3d_rec_create", "m3d_rec_destroy", "m3d_rec_last_error", "m3d_rec_set_initial", "m3d_rec_run", "m3d_rec_result",
 ]
 
 
@@ -92,6 +93,12 @@ def lib():
             "m3d_elastic_get_state": (ip, [vp, dp, dp]),
             "m3d_elastic_get_tuning": (ip, [vp, ipp, ipp, ipp]),
             "m3d_device_copy": (ip, [ct.c_double, ip, dp]),
+            "m3d_rec_create": (ip, [dp, ip, ip, ip, ct.POINTER(vp)]),
+            "m3d_rec_destroy": (ip, [vp]),
+            "m3d_rec_last_error": (cp, [vp]),
+            "m3d_rec_set_initial": (ip, [vp, dp]),
+            "m3d_rec_run": (ip, [vp, ip, ct.c_double, ct.c_uint64, ipp, ipp, ipp]),
+            "m3d_rec_result": (ip, [vp, dp, dp, ip, ipp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -397,3 +404,52 @@ class Batch:
         n, ms, vi = ct.c_int(0), ct.c_double(0), ct.c_double(0)
         self._chk(lib().m3d_batch_stats(self._h, ct.byref(n), ct.byref(ms), ct.byref(vi)))
         return {"count": n.value, "ms": ms.value, "voxel_iterations": vi.value}
+
+
+class recPlan:
+    """recPlan(double *TPCF, Nx, Ny, Nz) and runRec(n, b)
+    (include/reconstruction.hpp:55-74, src/rec/reconstruction.cpp:211-406):
+    phase retrieval of a two-phase microstructure from its two-point
+    correlation function, on the GPU in FP64.  mTilda and trackrealerrorA
+    are the reference's public results."""
+
+    def __init__(self, TPCF, Nx1=None, Ny1=None, Nz1=None):
+        t = np.ascontiguousarray(TPCF, dtype=np.float64)
+        shape = t.shape if Nx1 is None else (Nx1, Ny1, Nz1)
+        self.shape = tuple(int(v) for v in shape)
+        h = ct.c_void_p()
+        rc = lib().m3d_rec_create(_dp(t.reshape(-1)), *self.shape, ct.byref(h))
+        if rc != 0:
+            raise M3DError(rc, "m3d_rec_create failed")
+        self._h = h
+        self.mTilda = None
+        self.trackrealerrorA = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise M3DError(rc, lib().m3d_rec_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().m3d_rec_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_initial(self, m0):
+        self._m0 = np.ascontiguousarray(m0, dtype=np.float64).reshape(-1)
+        self._chk(lib().m3d_rec_set_initial(self._h, _dp(self._m0)))
+
+    def runRec(self, n, b, seed=1):
+        it, st, ss = ct.c_int(0), ct.c_int(0), ct.c_int(0)
+        self._chk(lib().m3d_rec_run(self._h, int(n), float(b), int(seed), ct.byref(it), ct.byref(st), ct.byref(ss)))
+        self.status, self.search_steps = st.value, ss.value
+        m = np.zeros(self.shape, dtype=np.complex128)
+        err = np.zeros(max(1, it.value), dtype=np.float64)
+        k = ct.c_int(0)
+        self._chk(lib().m3d_rec_result(self._h, m.view(np.float64).ctypes.data_as(ct.POINTER(ct.c_double)), _dp(err),
+                                       err.size, ct.byref(k)))
+        self.mTilda = m
+        self.trackrealerrorA = err[: k.value].copy()
+        return it.value

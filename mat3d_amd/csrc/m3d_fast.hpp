@@ -786,7 +786,10 @@ __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T
     constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
     rows_to_lds<T, L, W, NT, VEC>(sl, base, rm, wv);
     __syncthreads();
-    xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT>(g, p, p.ctl[s], prm_all + 2 * p.rve_of[s], sl, tw, red, s, o, kz0, nk, t);
+    // the Green step's reference medium (KIND 1 only: other users leave rve_of unset)
+    const T* prm = nullptr;
+    if constexpr (KIND == 1) prm = prm_all + 2 * p.rve_of[s];
+    xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT>(g, p, p.ctl[s], prm, sl, tw, red, s, o, kz0, nk, t);
     if constexpr (KIND != 2) lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
 }
 

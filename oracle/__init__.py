@@ -81,6 +81,14 @@ def lib():
             "m3o_th_info": (None, [vp, dp]),
             "m3o_th_history": (ip, [vp, dp, ip]),
             "m3o_th_clear_history": (None, [vp]),
+            "m3o_rec_uniform": (ct.c_double, [ct.c_uint64, ct.c_uint64, ct.c_uint64]),
+            "m3o_rec_mix": (ct.c_uint64, [ct.c_uint64]),
+            "m3o_c2c3": (None, [ip, ip, ip, dp, ip, ip]),
+            "m3o_rec_create": (vp, [dp, ip, ip, ip, ip]),
+            "m3o_rec_destroy": (None, [vp]),
+            "m3o_rec_set_initial": (None, [vp, dp]),
+            "m3o_rec_run": (ip, [vp, ip, ct.c_double, ct.c_uint64, ipp, ipp]),
+            "m3o_rec_result": (ip, [vp, dp, dp, ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -300,3 +308,57 @@ class Thermal:
         buf = np.zeros(4 * 4096)
         n = lib().m3o_th_history(self._h, _dp(buf), 4096)
         return buf[: 4 * n].reshape(n, 4)
+
+
+def c2c3(x: np.ndarray, sign: int, nthreads: int = 0) -> np.ndarray:
+    """Unnormalised complex 3D FFT of x[Nx, Ny, Nz] (sign -1 forward, +1 backward)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    Nx, Ny, Nz = a.shape
+    lib().m3o_c2c3(Nx, Ny, Nz, a.view(np.float64).ctypes.data_as(ct.POINTER(ct.c_double)), int(sign), _threads(nthreads))
+    return a
+
+
+def rec_uniform(seed: int, stream: int, i: int) -> float:
+    return lib().m3o_rec_uniform(seed, stream, i)
+
+
+def rec_mix(z: int) -> int:
+    return lib().m3o_rec_mix(z)
+
+
+class Rec:
+    """recPlan(double *TPCF, Nx, Ny, Nz) + runRec(n, beta)
+    (src/rec/reconstruction.cpp:211-243, 285-406)."""
+
+    def __init__(self, tpcf, nthreads=0):
+        t = np.ascontiguousarray(tpcf, dtype=np.float64)
+        self.shape = t.shape
+        self._tpcf = t
+        self._h = lib().m3o_rec_create(_dp(t), *t.shape, _threads(nthreads))
+        self._init = None
+
+    def close(self):
+        if self._h:
+            lib().m3o_rec_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_initial(self, m0):
+        self._init = np.ascontiguousarray(m0, dtype=np.float64).reshape(-1)
+        lib().m3o_rec_set_initial(self._h, _dp(self._init))
+
+    def run(self, n, beta, seed):
+        st, ctr = ct.c_int(0), ct.c_int(0)
+        it = lib().m3o_rec_run(self._h, int(n), float(beta), int(seed), ct.byref(st), ct.byref(ctr))
+        self.status, self.search_steps = st.value, ctr.value
+        return it
+
+    def result(self):
+        """(mTilda as complex [Nx, Ny, Nz], trackrealerrorA)"""
+        m = np.zeros(self.shape, dtype=np.complex128)
+        err = np.zeros(max(1, 100000), dtype=np.float64)
+        k = lib().m3o_rec_result(self._h, m.view(np.float64).ctypes.data_as(ct.POINTER(ct.c_double)), _dp(err),
+                                 err.size)
+        return m, err[:k].copy()

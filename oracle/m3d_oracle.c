@@ -1457,3 +1457,182 @@ int m3o_el_run_fixed(m3o_el *s, int w, int k)
     s->cap = old;
     return s->itNo;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Phase-retrieval reconstruction: recPlan(double *TPCF, Nx, Ny, Nz) and     */
+/* runRec(n, beta), src/rec/reconstruction.cpp:211-243 and :285-406.         */
+/* Random phases: the reference draws them with Eigen's setRandom (std::rand */
+/* through an unpinned Eigen version); here, as in the HIP path, a seeded    */
+/* counter-based stream: value i of stream k is splitmix64(seed + (k << 40)  */
+/* + i) scaled to [-1, 1).  Stream 0 is the start field, stream k the k-th   */
+/* re-randomisation (every 150 iterations).                                  */
+/* ------------------------------------------------------------------------ */
+static inline uint64_t rec_mix(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double m3o_rec_uniform(uint64_t seed, uint64_t stream, uint64_t i)
+{
+    return 2.0 * ((double)(rec_mix(seed + (stream << 40) + i) >> 11) * 0x1.0p-53) - 1.0;
+}
+
+uint64_t m3o_rec_mix(uint64_t z) { return rec_mix(z); }
+
+typedef struct {
+    int Nx, Ny, Nz;
+    long xyz;
+    int nthreads;
+    plan1d px, py, pz;
+    double *normC;
+    cplx *mT, *MT, *mH, *MH;
+    double *ang;
+    double *err;
+    int nerr, cap_err;
+    const double *init; /* optional start field (tests), else stream 0 */
+} m3o_rec;
+
+static void rec_fft3(m3o_rec *r, cplx *d, int sign)
+{
+    const long Nx = r->Nx, Ny = r->Ny, Nz = r->Nz;
+    fft_axis(&r->pz, d, Nx * Ny, Nz, 1, 1, sign, r->nthreads);
+    fft_axis(&r->py, d, Nx, Ny * Nz, Nz, Nz, sign, r->nthreads);
+    fft_axis(&r->px, d, 1, 0, Ny * Nz, Ny * Nz, sign, r->nthreads);
+}
+
+void m3o_c2c3(int Nx, int Ny, int Nz, double *ri, int sign, int nthreads)
+{
+    m3o_rec r;
+    memset(&r, 0, sizeof r);
+    r.Nx = Nx; r.Ny = Ny; r.Nz = Nz;
+    r.nthreads = nthreads > 0 ? nthreads : 1;
+    plan1d_init(&r.px, Nx); plan1d_init(&r.py, Ny); plan1d_init(&r.pz, Nz);
+    rec_fft3(&r, (cplx *)ri, sign);
+    plan1d_free(&r.px); plan1d_free(&r.py); plan1d_free(&r.pz);
+}
+
+/* recPlan(double *TPCF, ...), :211-243: normC = sqrt(|TPCF| xyz) */
+m3o_rec *m3o_rec_create(const double *tpcf, int Nx, int Ny, int Nz, int nthreads)
+{
+    m3o_rec *r = (m3o_rec *)xcalloc(1, sizeof(m3o_rec));
+    r->Nx = Nx; r->Ny = Ny; r->Nz = Nz;
+    r->xyz = (long)Nx * Ny * Nz;
+    r->nthreads = nthreads > 0 ? nthreads : 1;
+    plan1d_init(&r->px, Nx); plan1d_init(&r->py, Ny); plan1d_init(&r->pz, Nz);
+    r->normC = (double *)xcalloc((size_t)r->xyz, sizeof(double));
+    r->mT = (cplx *)xcalloc((size_t)r->xyz, sizeof(cplx));
+    r->MT = (cplx *)xcalloc((size_t)r->xyz, sizeof(cplx));
+    r->mH = (cplx *)xcalloc((size_t)r->xyz, sizeof(cplx));
+    r->MH = (cplx *)xcalloc((size_t)r->xyz, sizeof(cplx));
+    r->ang = (double *)xcalloc((size_t)r->xyz, sizeof(double));
+    for (long i = 0; i < r->xyz; ++i) r->normC[i] = sqrt(fabs(tpcf[i]) * (double)r->xyz);
+    return r;
+}
+
+void m3o_rec_destroy(m3o_rec *r)
+{
+    if (!r) return;
+    plan1d_free(&r->px); plan1d_free(&r->py); plan1d_free(&r->pz);
+    free(r->normC); free(r->mT); free(r->MT); free(r->mH); free(r->MH); free(r->ang); free(r->err);
+    free(r);
+}
+
+void m3o_rec_set_initial(m3o_rec *r, const double *m0) { r->init = m0; }
+
+/* runRec(n, beta), :285-406.  Returns the iterations done (the length of
+ * trackrealerrorA that was written); *status = 1 converged (error < 0.1),
+ * 2 iteration number exceeded; *ctr_out = iterations of the final volume
+ * fraction search. */
+int m3o_rec_run(m3o_rec *r, int n, double beta, uint64_t seed, int *status, int *ctr_out)
+{
+    const long xyz = r->xyz;
+    free(r->err);
+    r->err = (double *)xcalloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    r->nerr = 0;
+    /* ang = (setRandom + 1) * 0.5; mTilda = ang + 0i (:290-293) */
+    for (long s = 0; s < xyz; ++s) {
+        const double a = r->init ? r->init[s] : (m3o_rec_uniform(seed, 0, (uint64_t)s) + 1.0) * 0.5;
+        r->mT[s] = a;
+    }
+    int j = 0, i = 0, st = 0;
+    while (1) {
+        j += 1;
+        memcpy(r->MT, r->mT, sizeof(cplx) * (size_t)xyz);
+        rec_fft3(r, r->MT, -1); /* p1 */
+        if (i >= n) { st = 2; break; }
+        /* trackrealerrorA(i) = funcNormC(MTilda, normC) (reconstruction.hpp funcNormC) */
+        double e = 0;
+        for (long k = 0; k < xyz; ++k) e += fabs(sqrt(creal(r->MT[k]) * creal(r->MT[k]) + cimag(r->MT[k]) * cimag(r->MT[k])) - r->normC[k]);
+        e = e / (double)xyz;
+        r->err[i] = e;
+        r->nerr = i + 1;
+        if (e < 0.1) { st = 1; break; }
+        for (long s = 0; s < xyz; ++s) r->ang[s] = carg(r->MT[s]);
+        if (remainder((double)(i + 1), 150.0) == 0) {
+            /* ang.setRandom; mHat = ang; p3; ang2 = arg(MHat) .3; ang1 = arg(MTilda) .7 (:321-333) */
+            const uint64_t k = (uint64_t)((i + 1) / 150);
+            for (long s = 0; s < xyz; ++s) r->mH[s] = m3o_rec_uniform(seed, k, (uint64_t)s);
+            memcpy(r->MH, r->mH, sizeof(cplx) * (size_t)xyz);
+            rec_fft3(r, r->MH, -1);
+            for (long s = 0; s < xyz; ++s) r->ang[s] = carg(r->MH[s]) * .3 + carg(r->MT[s]) * .7;
+        }
+        /* MHat = normC e^{i ang}; p4; / xyz (:335-346) */
+        for (long s = 0; s < xyz; ++s) r->MH[s] = r->normC[s] * cos(r->ang[s]) + I * (r->normC[s] * sin(r->ang[s]));
+        memcpy(r->mH, r->MH, sizeof(cplx) * (size_t)xyz);
+        rec_fft3(r, r->mH, +1);
+        for (long s = 0; s < xyz; ++s) r->mH[s] = creal(r->mH[s]) / (double)xyz + I * (cimag(r->mH[s]) / (double)xyz);
+        if (j < 100) {
+            /* :348-356 */
+            for (long s = 0; s < xyz; ++s) {
+                double re = creal(r->mH[s]);
+                if (re < 0) { re = creal(r->mT[s]) - beta * re; r->mH[s] = re; }
+                if (re > 1) { re = creal(r->mT[s]) - beta * (re - 1); r->mH[s] = re; }
+                if (re > 1) { re = 1; r->mH[s] = re; }
+            }
+            memcpy(r->mT, r->mH, sizeof(cplx) * (size_t)xyz);
+        }
+        if (j >= 100) {
+            /* :358-366 */
+            for (long s = 0; s < xyz; ++s) {
+                const double re = creal(r->mH[s]);
+                if (re < 0) r->mH[s] = 0;
+                if (re > 1) r->mH[s] = 1;
+            }
+            if (j > 200) j = 0;
+            memcpy(r->mT, r->mH, sizeof(cplx) * (size_t)xyz);
+        }
+        i += 1;
+    }
+    /* volume fraction search (:374-402) */
+    int ctr = 0;
+    double x = 0, vol_dif = 0, thr = 0.5;
+    while (1) {
+        thr = .5 - x;
+        double sum = 0;
+        for (long s = 0; s < xyz; ++s) {
+            r->ang[s] = (creal(r->mT[s]) < thr) ? 0.0 : 1.0;
+            sum += r->ang[s];
+        }
+        vol_dif = (sum / (1. * xyz) - r->normC[0] / (1. * xyz));
+        if (vol_dif > 0) x -= .001;
+        else x += .001;
+        if (fabs(vol_dif) < .0001 || ctr > 200) break;
+        ctr += 1;
+    }
+    for (long s = 0; s < xyz; ++s) r->mT[s] = r->ang[s] + I * cimag(r->mT[s]);
+    if (status) *status = st;
+    if (ctr_out) *ctr_out = ctr;
+    return r->nerr;
+}
+
+/* mTilda as interleaved (re, im) and trackrealerrorA */
+int m3o_rec_result(const m3o_rec *r, double *mtilda_ri, double *err, int maxn)
+{
+    if (mtilda_ri) memcpy(mtilda_ri, r->mT, sizeof(cplx) * (size_t)r->xyz);
+    const int n = r->nerr < maxn ? r->nerr : maxn;
+    if (err && n > 0) memcpy(err, r->err, sizeof(double) * (size_t)n);
+    return r->nerr;
+}
