@@ -100,9 +100,9 @@ def main():
                     help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
                          "reference for HBM byte counters, then exit")
     ap.add_argument("--workload", default="fibres256",
-                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64"],
-                    help="fibres256 is the headline line; the others measure the batch, thermal and "
-                         "J2 paths on their own workloads")
+                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64", "rec128"],
+                    help="fibres256 is the headline line; the others measure the batch, thermal, J2, "
+                         "basic-scheme and reconstruction paths on their own workloads")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -381,8 +381,60 @@ def bench_basic64(args):
           {"iterations": its, "C_eff": s.CHomo.tolist()})
 
 
+def bench_rec128(args):
+    """Phase-retrieval reconstruction (recPlan / runRec) of a 128^3 two-phase
+    structure (spheres of radius 6 at 25%, seed 3) from its two-point
+    correlation function: 600 iterations (beta 0.8, random stream seed 1;
+    re-randomisations at 150, 300, 450) in FP64 on the GPU, beside the
+    oracle on the host's cores for a few iterations."""
+    import mat3d_amd as m
+    from mat3d_amd import gen
+
+    n = 128
+    N = n**3
+    rve = (gen.spheres(n, 6, 0.25, seed=3) == 0).astype(float)
+    tpcf = np.abs(np.fft.fftn(rve)) ** 2 / N
+    g = m.recPlan(tpcf)
+    g.runRec(20, 0.8, 1)  # warm-up: graph capture, first launches
+    iters = 600
+    vi = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it = g.runRec(iters, 0.8, 1)
+        vi += N * it
+    dt = time.perf_counter() - t0
+    value = vi / dt
+    # per iteration: forward and inverse 3D FFT (read + write 16 B each),
+    # phase (read MT, normC; write MH), projection (read MH, mT; write mT)
+    B = 32 + 40 + 32 + 48
+    it_gbs = B * value / 1e9
+    print(f"[bench] rec128: {value:.4e} voxel-it/s, {it} iterations in {dt / args.steps:.3f} s, final error "
+          f"{g.trackrealerrorA[-1]:.4f}, volume fraction {g.mTilda.real.mean():.5f} (target {rve.mean():.5f})",
+          flush=True)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.Rec(tpcf, nthreads=threads)
+        o.run(1, 0.8, 1)
+        k = 3
+        t1 = time.perf_counter()
+        o.run(k, 0.8, 1)
+        tc = time.perf_counter() - t1
+        cpu = {"value": N * k / tc, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
+               "sample": f"128^3, {k} iterations of runRec after one warm-up ({tc:.1f}s; oracle's own FP64 FFT)"}
+        print(f"[bench] cpu oracle: {N * k / tc:.4e} voxel-it/s on {threads} threads", flush=True)
+    _line(args, "voxel-iters/s, 128^3 phase-retrieval reconstruction", value, "voxel-iters/s",
+          dt / args.steps * 1e3, "f64", "128^3 recPlan/runRec, 600 iterations, beta 0.8",
+          {"iterations": it, "final_error": float(g.trackrealerrorA[-1]),
+           "roofline_iteration": {"bytes_per_voxel_iter": B, "achieved": round(it_gbs, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(it_gbs / HBM_PEAK_GBS, 4)},
+           "cpu_baseline": cpu})
+
+
 OTHER = {"voronoi_batch": bench_voronoi_batch, "thermal512": bench_thermal512, "j2_192": bench_j2_192,
-         "basic64": bench_basic64}
+         "basic64": bench_basic64, "rec128": bench_rec128}
 
 
 if __name__ == "__main__":
