@@ -120,7 +120,7 @@ def main():
 
     n = args.size
     t0 = time.perf_counter()
-    ids = gen.fibres(n, 4, 0.30, seed=2, length=64)
+    ids = gen.fibres(n, 4, 0.30, seed=2, length=64, device=True)
     t_gen = time.perf_counter() - t0
     mat = gen.MAT_C2
     N = n**3
@@ -257,7 +257,7 @@ def bench_voronoi_batch(args):
     from mat3d_amd import gen
 
     t0 = time.perf_counter()
-    probs = gen.c3_batch(64)
+    probs = gen.c3_batch(64, device=True)
     print(f"[bench] generated 64 Voronoi RVEs in {time.perf_counter() - t0:.1f}s, "
           f"sizes {sorted(set(p[0] for p in probs))}", flush=True)
     b = m.Batch("fp32_polish")
@@ -288,7 +288,7 @@ def bench_thermal512(args):
 
     n = 512
     t0 = time.perf_counter()
-    ids = gen.c5_rve(n)
+    ids = gen.c5_rve(n, device=True)
     print(f"[bench] generated {n}^3 three-phase RVE in {time.perf_counter() - t0:.1f}s", flush=True)
     k = np.array([0.01, 1.0, 1000.0])  # phase 0 pores, 1 matrix, 2 inclusions
     s = m.FFT_analysis3d_thermal(n, n, n, precision="fp32_polish")
@@ -318,7 +318,7 @@ def bench_j2_192(args):
     from mat3d_amd import gen
 
     n = 192
-    ids = gen.c4_rve(n)
+    ids = gen.c4_rve(n, device=True)
     X0 = np.eye(6)[0]
 
     def run(precision, steps, **opt):
@@ -361,7 +361,7 @@ def bench_basic64(args):
     from mat3d_amd import gen
 
     n = 64
-    ids = gen.c1_rve(n)
+    ids = gen.c1_rve(n, device=True)
     s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
     s.set_option("scheme", 1)
     s.material(gen.MAT_C1, ids)
@@ -392,7 +392,7 @@ def bench_rec128(args):
 
     n = 128
     N = n**3
-    rve = (gen.spheres(n, 6, 0.25, seed=3) == 0).astype(float)
+    rve = (gen.spheres(n, 6, 0.25, seed=3, device=True) == 0).astype(float)
     tpcf = np.abs(np.fft.fftn(rve)) ** 2 / N
     g = m.recPlan(tpcf)
     g.runRec(20, 0.8, 1)  # warm-up: graph capture, first launches

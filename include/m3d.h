@@ -272,6 +272,34 @@ int m3d_rec_run(m3d_rec *r, int n, double beta, uint64_t seed, int *iterations, 
  * include/reconstruction.hpp:73-74. */
 int m3d_rec_result(m3d_rec *r, double *mtilda_ri, double *err, int maxn, int *nerr);
 
+/* ---------------------------------------------------------------------
+ * Voxel work of the seeded structure generators (mat3d_amd/gen.py; no
+ * reference counterpart: the reference reads RVE files).  The random draws
+ * and placement decisions stay with the caller; these rasterise on the GPU
+ * with the generators' own tests and operation order, so the phase ids are
+ * theirs bit for bit.  ids: Nx*Ny*Nz int32, index (x*Ny + y)*Nz + z.
+ * ------------------------------------------------------------------- */
+const char *m3d_gen_last_error(void);
+
+/* ids = fill, then every ball {cx, cy, cz, r, phase} (5 int32 each) stamps
+ * its phase on the voxels c + o with |o|^2 <= r^2 (periodic).  Balls must not
+ * share voxels (random sequential addition guarantees it). */
+int m3d_gen_balls(int nx, int ny, int nz, const int32_t *balls, int nballs, int32_t fill, int32_t *ids);
+
+/* Straight finite cylinders stamped onto ids (in/out) in batches of `batch`
+ * until the count of `incl` voxels reaches `target`: cylinder k = {axis v
+ * (unit, 3), centre c (3), box lo (3), box hi (3)} (12 doubles); voxel a of the
+ * box with X = a - c, t = X.v: |t| <= half and |X|^2 - t^2 <= radius^2.
+ * used = cylinders stamped, count = final count (stops early when the list
+ * runs out: the caller draws more and calls again). */
+int m3d_gen_cylinders(int nx, int ny, int nz, const double *cyl, int ncyl, double half, double radius, int32_t incl,
+                      int batch, long long target, int32_t *ids, int *used, long long *count);
+
+/* Periodic Voronoi cells: ids = index of the nearest of the g seeds (3
+ * doubles each, inside the box) under the minimum-image distance, the first
+ * seed on exact ties. */
+int m3d_gen_voronoi(int nx, int ny, int nz, const double *pts, int g, int32_t *ids);
+
 #ifdef __cplusplus
 }
 #endif
