@@ -300,6 +300,42 @@ int m3d_gen_cylinders(int nx, int ny, int nz, const double *cyl, int ncyl, doubl
  * seed on exact ties. */
 int m3d_gen_voronoi(int nx, int ny, int nz, const double *pts, int g, int32_t *ids);
 
+/* ---------------------------------------------------------------------
+ * FFT_Analysis2D (include/FFT_analysis2D.hpp, src/2d/FFT_analysis2D.cpp),
+ * FP64: basic scheme on complex 2D fields [E11, E22, E12] (tensor shear),
+ * 25 fixed iterations per run.  Fields: Nx*Ny points, index y + x*Ny, three
+ * interleaved complex components (re, im).  Nx and Ny: 32, 48, 64, 96, 128,
+ * 192, 256 or 512.
+ * ------------------------------------------------------------------- */
+typedef struct m3d_2d m3d_2d;
+
+/* FFT_Analysis2D(int Nx1, int Ny1) -- FFT_analysis2D.hpp:82, .cpp:3-51.  The
+ * fields start at zero (the reference leaves them uninitialised). */
+int m3d_2d_create(int nx, int ny, m3d_2d **out);
+int m3d_2d_destroy(m3d_2d *h);
+const char *m3d_2d_last_error(m3d_2d *h);
+
+/* material(double *properties, MatrixDdd &RVE1) -- .hpp:89, .cpp:189-221:
+ * properties {E matrix, nu matrix, E inclusion, nu inclusion}; rve Nx*Ny
+ * values, 1 = matrix, anything else = inclusion. */
+int m3d_2d_material(m3d_2d *h, const double *properties, const double *rve);
+
+/* setLoad(MatrixCdd &E1) -- .hpp:92, .cpp:223-230: E (3 complex, re/im). */
+int m3d_2d_set_load(m3d_2d *h, const double *E_ri);
+
+/* runAnalysis() -- .hpp:95, .cpp:232-246: 25 iterations from the current
+ * fields; itNo (may be NULL) = iterations run by this handle so far. */
+int m3d_2d_run(m3d_2d *h, int *itNo);
+
+/* homogenization() -- .hpp:97, .cpp:248-272: three runs with E(w) = 0.005
+ * from the current fields (the reference does not reset them), then CHomo
+ * = epHomo^-1 sigHomo; all 3x3 column-major (may be NULL).  The reference's
+ * report file "Homogenized_Properties" is not written. */
+int m3d_2d_homogenization(m3d_2d *h, double *CHomo, double *epHomo, double *sigHomo);
+
+/* public epsilon / sigma (3 x Nx*Ny complex) -- .hpp:28-29 */
+int m3d_2d_get_fields(m3d_2d *h, double *eps_ri, double *sig_ri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,8 @@ EXPORTS = [
     "m3d_batch_stats", "m3d_batch_set_option", "m3d_batch_last_error",
     "m3d_rec_create", "m3d_rec_destroy", "m3d_rec_last_error", "m3d_rec_set_initial", "m3d_rec_run", "m3d_rec_result",
     "m3d_gen_last_error", "m3d_gen_balls", "m3d_gen_cylinders", "m3d_gen_voronoi",
+    "m3d_2d_create", "m3d_2d_destroy", "m3d_2d_last_error", "m3d_2d_material", "m3d_2d_set_load", "m3d_2d_run",
+    "m3d_2d_homogenization", "m3d_2d_get_fields",
 ]
 
 
@@ -105,6 +107,14 @@ def lib():
             "m3d_gen_cylinders": (ip, [ip, ip, ip, dp, ip, ct.c_double, ct.c_double, ct.c_int32, ip, ct.c_longlong,
                                        i32p, ipp, ct.POINTER(ct.c_longlong)]),
             "m3d_gen_voronoi": (ip, [ip, ip, ip, dp, ip, i32p]),
+            "m3d_2d_create": (ip, [ip, ip, ct.POINTER(vp)]),
+            "m3d_2d_destroy": (ip, [vp]),
+            "m3d_2d_last_error": (cp, [vp]),
+            "m3d_2d_material": (ip, [vp, dp, dp]),
+            "m3d_2d_set_load": (ip, [vp, dp]),
+            "m3d_2d_run": (ip, [vp, ipp]),
+            "m3d_2d_homogenization": (ip, [vp, dp, dp, dp]),
+            "m3d_2d_get_fields": (ip, [vp, dp, dp]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -459,3 +469,60 @@ class recPlan:
         self.mTilda = m
         self.trackrealerrorA = err[: k.value].copy()
         return it.value
+
+
+class FFT_Analysis2D:
+    """FFT_Analysis2D(Nx1, Ny1) (include/FFT_analysis2D.hpp:82-98,
+    src/2d/FFT_analysis2D.cpp): the 2D basic scheme on complex fields
+    [E11, E22, E12], 25 fixed iterations per runAnalysis, on the GPU (FP64)."""
+
+    def __init__(self, Nx1, Ny1):
+        self.Nx, self.Ny = int(Nx1), int(Ny1)
+        self.xy = self.Nx * self.Ny
+        h = ct.c_void_p()
+        rc = lib().m3d_2d_create(self.Nx, self.Ny, ct.byref(h))
+        if rc != 0:
+            raise M3DError(rc, "m3d_2d_create failed")
+        self._h = h
+        self.itNo = 0
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise M3DError(rc, lib().m3d_2d_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().m3d_2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def material(self, properties, RVE1):
+        q = np.ascontiguousarray(properties, dtype=np.float64)
+        r = np.ascontiguousarray(RVE1, dtype=np.float64).reshape(-1)
+        self._chk(lib().m3d_2d_material(self._h, _dp(q), _dp(r)))
+
+    def setLoad(self, E1):
+        e = np.ascontiguousarray(np.asarray(E1, dtype=np.complex128).reshape(3))
+        self._chk(lib().m3d_2d_set_load(self._h, _dp(e.view(np.float64))))
+
+    def runAnalysis(self):
+        it = ct.c_int(0)
+        self._chk(lib().m3d_2d_run(self._h, ct.byref(it)))
+        self.itNo = it.value
+
+    def homogenization(self):
+        C, ep, sg = np.zeros(9), np.zeros(9), np.zeros(9)
+        self._chk(lib().m3d_2d_homogenization(self._h, _dp(C), _dp(ep), _dp(sg)))
+        self.CHomo = C.reshape(3, 3, order="F")
+        self.epHomo = ep.reshape(3, 3, order="F")
+        self.sigHomo = sg.reshape(3, 3, order="F")
+        return self.CHomo
+
+    def fields(self):
+        """(epsilon, sigma) as complex arrays (Nx*Ny, 3)"""
+        e = np.zeros((self.xy, 3), dtype=np.complex128)
+        s = np.zeros((self.xy, 3), dtype=np.complex128)
+        self._chk(lib().m3d_2d_get_fields(self._h, _dp(e.view(np.float64)), _dp(s.view(np.float64))))
+        return e, s

@@ -89,6 +89,13 @@ def lib():
             "m3o_rec_set_initial": (None, [vp, dp]),
             "m3o_rec_run": (ip, [vp, ip, ct.c_double, ct.c_uint64, ipp, ipp]),
             "m3o_rec_result": (ip, [vp, dp, dp, ip]),
+            "m3o_2d_create": (vp, [ip, ip, ip]),
+            "m3o_2d_destroy": (None, [vp]),
+            "m3o_2d_material": (None, [vp, dp, dp]),
+            "m3o_2d_set_load": (None, [vp, dp]),
+            "m3o_2d_run": (None, [vp]),
+            "m3o_2d_homogenization": (None, [vp, dp, dp, dp]),
+            "m3o_2d_fields": (None, [vp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -362,3 +369,44 @@ class Rec:
         k = lib().m3o_rec_result(self._h, m.view(np.float64).ctypes.data_as(ct.POINTER(ct.c_double)), _dp(err),
                                  err.size)
         return m, err[:k].copy()
+
+
+class Elastic2D:
+    """FFT_Analysis2D (src/2d/FFT_analysis2D.cpp): basic scheme on complex 2D
+    fields [E11, E22, E12], 25 fixed iterations per runAnalysis."""
+
+    def __init__(self, Nx, Ny, nthreads=0):
+        self.shape = (Nx, Ny)
+        self._h = lib().m3o_2d_create(Nx, Ny, _threads(nthreads))
+
+    def close(self):
+        if self._h:
+            lib().m3o_2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def material(self, props, rve):
+        self._q = np.ascontiguousarray(props, dtype=np.float64)
+        self._r = np.ascontiguousarray(rve, dtype=np.float64).reshape(-1)
+        lib().m3o_2d_material(self._h, _dp(self._q), _dp(self._r))
+
+    def set_load(self, E):
+        e = np.ascontiguousarray(np.asarray(E, dtype=np.complex128).reshape(3))
+        lib().m3o_2d_set_load(self._h, _dp(e.view(np.float64)))
+
+    def run(self):
+        lib().m3o_2d_run(self._h)
+
+    def homogenization(self):
+        C, ep, sg = np.zeros(9), np.zeros(9), np.zeros(9)
+        lib().m3o_2d_homogenization(self._h, _dp(C), _dp(ep), _dp(sg))
+        return C.reshape(3, 3, order="F"), ep.reshape(3, 3, order="F"), sg.reshape(3, 3, order="F")
+
+    def fields(self):
+        n = self.shape[0] * self.shape[1]
+        e = np.zeros((n, 3), dtype=np.complex128)
+        s = np.zeros((n, 3), dtype=np.complex128)
+        lib().m3o_2d_fields(self._h, _dp(e.view(np.float64)), _dp(s.view(np.float64)))
+        return e, s

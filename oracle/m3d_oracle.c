@@ -1636,3 +1636,180 @@ int m3o_rec_result(const m3o_rec *r, double *mtilda_ri, double *err, int maxn)
     if (err && n > 0) memcpy(err, r->err, sizeof(double) * (size_t)n);
     return r->nerr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FFT_Analysis2D (src/2d/FFT_analysis2D.cpp): basic scheme on complex 2D    */
+/* fields [E11, E22, E12] (tensor shear), 25 fixed iterations.               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny;
+    long xy;
+    int nthreads;
+    plan1d px, py;
+    double Cm[9], Ci[9], C0[9], la0, mu0;
+    double *rve;
+    cplx *eps, *sig, *sigOld, *tau, *t;
+    double *GF;
+    cplx E[3];
+    int itNo;
+} m3o_2d;
+
+m3o_2d *m3o_2d_create(int Nx, int Ny, int nthreads)
+{
+    m3o_2d *s = (m3o_2d *)xcalloc(1, sizeof(m3o_2d));
+    s->Nx = Nx; s->Ny = Ny; s->xy = (long)Nx * Ny;
+    s->nthreads = nthreads > 0 ? nthreads : 1;
+    plan1d_init(&s->px, Nx); plan1d_init(&s->py, Ny);
+    s->rve = (double *)xcalloc((size_t)s->xy, sizeof(double));
+    s->eps = (cplx *)xcalloc((size_t)(3 * s->xy), sizeof(cplx));
+    s->sig = (cplx *)xcalloc((size_t)(3 * s->xy), sizeof(cplx));
+    s->sigOld = (cplx *)xcalloc((size_t)(3 * s->xy), sizeof(cplx));
+    s->tau = (cplx *)xcalloc((size_t)(3 * s->xy), sizeof(cplx));
+    s->t = (cplx *)xcalloc((size_t)(3 * s->xy), sizeof(cplx));
+    s->GF = (double *)xcalloc((size_t)(6 * s->xy), sizeof(double));
+    return s;
+}
+
+void m3o_2d_destroy(m3o_2d *s)
+{
+    if (!s) return;
+    plan1d_free(&s->px); plan1d_free(&s->py);
+    free(s->rve); free(s->eps); free(s->sig); free(s->sigOld); free(s->tau); free(s->t); free(s->GF);
+    free(s);
+}
+
+/* 2D complex FFT of 3 interleaved components (p3 / p4, :29-50) */
+static void fft2_3(m3o_2d *s, cplx *d, int sign)
+{
+    fft_axis(&s->py, d, s->Nx, (long)s->Ny * 3, 3, 3, sign, s->nthreads);
+    fft_axis(&s->px, d, 1, 0, (long)s->Ny * 3, (long)s->Ny * 3, sign, s->nthreads);
+}
+
+/* material(properties, RVE1), :189-221 */
+void m3o_2d_material(m3o_2d *s, const double *q, const double *rve)
+{
+    const double Em = q[0], vm = q[1], Ei = q[2], vi = q[3];
+    const double mum = Em / (1 + vm) / 2, lam = vm * Em / (1 + vm) / (1 - 2 * vm);
+    const double mui = Ei / (1 + vi) / 2, lai = vi * Ei / (1 + vi) / (1 - 2 * vi);
+    const double cm[9] = {lam + 2 * mum, lam, 0, lam, lam + 2 * mum, 0, 0, 0, 2 * mum};
+    const double ci[9] = {lai + 2 * mui, lai, 0, lai, lai + 2 * mui, 0, 0, 0, 2 * mui};
+    const double alf = 1;
+    for (int k = 0; k < 9; ++k) {
+        s->Cm[k] = cm[k];
+        s->Ci[k] = ci[k];
+        s->C0[k] = alf * (cm[k] + ci[k]) / 2.;
+    }
+    s->la0 = alf * (lai + lam) / 2.;
+    s->mu0 = alf * (mui + mum) / 2.;
+    memcpy(s->rve, rve, sizeof(double) * (size_t)s->xy);
+}
+
+/* setLoad(E1), :223-230 */
+void m3o_2d_set_load(m3o_2d *s, const double *E_ri)
+{
+    for (int c = 0; c < 3; ++c) s->E[c] = E_ri[2 * c] + I * E_ri[2 * c + 1];
+    for (long i = 0; i < s->xy; ++i)
+        for (int c = 0; c < 3; ++c) s->eps[3 * i + c] = s->E[c];
+}
+
+/* setGF, :53-93 (x > (Nx-1)/2 maps to -(Ny - x), as the reference) */
+static void m3o_2d_setGF(m3o_2d *s)
+{
+    const double d1 = 1 / (4 * s->mu0);
+    const double d2 = (s->la0 + s->mu0) / (s->mu0 * (s->la0 + 2 * s->mu0));
+    static const int q[6][4] = {{0, 0, 0, 0}, {0, 0, 1, 1}, {0, 0, 0, 1}, {1, 1, 1, 1}, {1, 1, 0, 1}, {0, 1, 0, 1}};
+    for (int x = 0; x < s->Nx; x++)
+        for (int y = 0; y < s->Ny; y++) {
+            const long cc = ((long)y + (long)x * s->Ny) * 6;
+            const int x1 = (x <= (s->Nx - 1) / 2) ? x : -(s->Ny - x);
+            const int y1 = (y <= (s->Ny - 1) / 2) ? y : -(s->Ny - y);
+            const double zet[2] = {x1 * 1.0, y1 * 1.0};
+            const double n2 = zet[0] * zet[0] + zet[1] * zet[1];
+            for (int r = 0; r < 6; r++) {
+                const int i = q[r][0], j = q[r][1], k = q[r][2], h = q[r][3];
+                s->GF[cc + r] = (d1 / n2) * (dlt(k, i) * zet[h] * zet[j] + dlt(h, i) * zet[k] * zet[j] +
+                                             dlt(k, j) * zet[h] * zet[i] + dlt(h, j) * zet[k] * zet[i]) -
+                                (d2 * zet[i] * zet[j] * zet[k] * zet[h]) / (n2 * n2);
+            }
+        }
+}
+
+/* runAnalysis, :232-246: 25 x {multipCx, tau, FFT_tau, multiplyGF,
+ * FFTBCKWRD_epsilon, convergenceCheck}.  convergenceCheck only prints (and
+ * its cc reads past the 2-element frequency vector); its itNo is kept. */
+void m3o_2d_run(m3o_2d *s)
+{
+    m3o_2d_setGF(s);
+    const long xy = s->xy;
+    static const int w[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int it = 0; it < 25; ++it) {
+        /* multipCx :96-106 */
+        memcpy(s->sigOld, s->sig, sizeof(cplx) * (size_t)(3 * xy));
+        for (long i = 0; i < xy; ++i) {
+            const double *C = (s->rve[i] == 1) ? s->Cm : s->Ci;
+            const cplx *e = s->eps + 3 * i;
+            for (int r = 0; r < 3; ++r) s->sig[3 * i + r] = C[3 * r] * e[0] + C[3 * r + 1] * e[1] + C[3 * r + 2] * e[2];
+        }
+        /* tau = sigma - C0 epsilon (:240) */
+        for (long i = 0; i < xy; ++i) {
+            const cplx *e = s->eps + 3 * i;
+            for (int r = 0; r < 3; ++r)
+                s->tau[3 * i + r] = s->sig[3 * i + r] - (s->C0[3 * r] * e[0] + s->C0[3 * r + 1] * e[1] + s->C0[3 * r + 2] * e[2]);
+        }
+        /* FFT_tau :108-112 */
+        memcpy(s->t, s->tau, sizeof(cplx) * (size_t)(3 * xy));
+        fft2_3(s, s->t, -1);
+        /* multiplyGF :114-141 (in place: every output reads its own frequency) */
+        for (long i = 1; i < xy; i++) {
+            const long u = i * 3, d = i * 6;
+            const cplx a0 = s->t[u], a1 = s->t[u + 1], a2 = s->t[u + 2];
+            for (int j = 0; j < 3; j++) {
+                const double re = -(s->GF[d + w[j][0]] * creal(a0) + s->GF[d + w[j][1]] * creal(a1) +
+                                    s->GF[d + w[j][2]] * creal(a2) * 2);
+                const double im = -(s->GF[d + w[j][0]] * cimag(a0) + s->GF[d + w[j][1]] * cimag(a1) +
+                                    s->GF[d + w[j][2]] * cimag(a2) * 2);
+                s->t[u + j] = re + I * im;
+            }
+        }
+        for (int c = 0; c < 3; ++c) s->t[c] = creal(s->E[c]) * xy + I * (cimag(s->E[c]) * xy);
+        /* FFTBCKWRD_epsilon :143-148 */
+        fft2_3(s, s->t, +1);
+        for (long k = 0; k < 3 * xy; ++k) s->eps[k] = creal(s->t[k]) / xy + I * (cimag(s->t[k]) / xy);
+        s->itNo += 1;
+    }
+}
+
+/* homogenization, :248-272: three load cases E(w) = 0.005 from the current
+ * fields (the reference does not reset epsilon between them), means of the
+ * real parts, CHomo.col(k) = epHomo^-1 sigHomo.col(k).  Matrices column-major. */
+void m3o_2d_homogenization(m3o_2d *s, double *CHomo, double *epHomo, double *sigHomo)
+{
+    double ep[9], sg[9];
+    for (int w = 0; w < 3; ++w) {
+        for (int c = 0; c < 3; ++c) s->E[c] = (c == w) ? 0.005 : 0.0;
+        m3o_2d_run(s);
+        double e[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        for (long t = 0; t < s->xy; ++t)
+            for (int c = 0; c < 3; ++c) {
+                e[c] += creal(s->eps[3 * t + c]);
+                g[c] += creal(s->sig[3 * t + c]);
+            }
+        for (int c = 0; c < 3; ++c) {
+            ep[w + 3 * c] = e[c] / s->xy; /* epHomo(w, c) */
+            sg[c + 3 * w] = g[c] / s->xy; /* sigHomo(c, w) */
+        }
+    }
+    double inv[9];
+    mat_inv(ep, inv, 3);
+    double C[9];
+    mat_mul(inv, sg, C, 3);
+    if (CHomo) memcpy(CHomo, C, sizeof C);
+    if (epHomo) memcpy(epHomo, ep, sizeof ep);
+    if (sigHomo) memcpy(sigHomo, sg, sizeof sg);
+}
+
+void m3o_2d_fields(const m3o_2d *s, double *eps_ri, double *sig_ri)
+{
+    if (eps_ri) memcpy(eps_ri, s->eps, sizeof(cplx) * (size_t)(3 * s->xy));
+    if (sig_ri) memcpy(sig_ri, s->sig, sizeof(cplx) * (size_t)(3 * s->xy));
+}
