@@ -100,9 +100,9 @@ def main():
                     help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
                          "reference for HBM byte counters, then exit")
     ap.add_argument("--workload", default="fibres256",
-                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64", "rec128"],
+                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64", "rec128", "plate512"],
                     help="fibres256 is the headline line; the others measure the batch, thermal, J2, "
-                         "basic-scheme and reconstruction paths on their own workloads")
+                         "basic-scheme, reconstruction and 2D paths on their own workloads")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -433,8 +433,57 @@ def bench_rec128(args):
            "cpu_baseline": cpu})
 
 
+def bench_plate512(args):
+    """FFT_Analysis2D homogenization of a 512^2 plate with 40 random discs
+    (the example's moduli: matrix 68.9 GPa / 0.35, inclusion 400 GPa / 0.23):
+    3 load cases x 25 iterations in FP64 on the GPU, beside the oracle."""
+    import mat3d_amd as m
+
+    n = 512
+    rng = np.random.default_rng(3)
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    rve = np.ones((n, n))
+    for _ in range(40):
+        cx, cy, rad = rng.uniform(0, n), rng.uniform(0, n), rng.uniform(10, 30)
+        dx = np.minimum(np.abs(x - cx), n - np.abs(x - cx))
+        dy = np.minimum(np.abs(y - cy), n - np.abs(y - cy))
+        rve[dx * dx + dy * dy <= rad * rad] = 0
+    rve = rve.reshape(-1)
+    q = [68.9e9, 0.35, 400e9, 0.23]
+    g = m.FFT_Analysis2D(n, n)
+    g.material(q, rve)
+    g.setLoad([0, 0, 0])
+    g.homogenization()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.setLoad([0, 0, 0])
+        C = g.homogenization()
+    dt = time.perf_counter() - t0
+    vi = n * n * 75 * args.steps
+    print(f"[bench] plate512: {vi / dt:.4e} point-iterations/s, {dt / args.steps * 1e3:.1f} ms per homogenization, "
+          f"C11 {C[0, 0]:.4e}", flush=True)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.Elastic2D(n, n, nthreads=threads)
+        o.material(q, rve)
+        o.set_load([0, 0, 0])
+        t1 = time.perf_counter()
+        Co, _, _ = o.homogenization()
+        tc = time.perf_counter() - t1
+        cpu = {"value": n * n * 75 / tc, "unit": "point-iterations/s", "cores": threads, "kind": "port",
+               "sample": f"one 512^2 homogenization (3 x 25 iterations, {tc:.1f}s; oracle's own FP64 FFT)"}
+        print(f"[bench] cpu oracle: {n * n * 75 / tc:.4e} point-iterations/s on {threads} threads, "
+              f"CHomo rel diff {np.abs(C - Co).max() / np.abs(Co).max():.2e}", flush=True)
+    _line(args, "point-iterations/s, 512^2 FFT_Analysis2D homogenization", vi / dt, "point-iterations/s",
+          dt / args.steps * 1e3, "f64", "512^2 plate, 40 discs, 3 load cases x 25 iterations",
+          {"CHomo": C.tolist(), "cpu_baseline": cpu})
+
+
 OTHER = {"voronoi_batch": bench_voronoi_batch, "thermal512": bench_thermal512, "j2_192": bench_j2_192,
-         "basic64": bench_basic64, "rec128": bench_rec128}
+         "basic64": bench_basic64, "rec128": bench_rec128, "plate512": bench_plate512}
 
 
 if __name__ == "__main__":
