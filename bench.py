@@ -100,9 +100,10 @@ def main():
                     help="only run a fill and three device copies of GIB GiB (torch kernels), the known-size "
                          "reference for HBM byte counters, then exit")
     ap.add_argument("--workload", default="fibres256",
-                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64", "rec128", "plate512"],
+                    choices=["fibres256", "voronoi_batch", "thermal512", "j2_192", "basic64", "rec128", "plate512",
+                             "perc256"],
                     help="fibres256 is the headline line; the others measure the batch, thermal, J2, "
-                         "basic-scheme, reconstruction and 2D paths on their own workloads")
+                         "basic-scheme, reconstruction, 2D and percolation paths on their own workloads")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -482,8 +483,81 @@ def bench_plate512(args):
           {"CHomo": C.tolist(), "cpu_baseline": cpu})
 
 
+def bench_perc256(args):
+    """percolation (runPercolationAnalysis) of the fibre phase of the 256^3
+    fibre structure of the headline line (radius 4, 30%, seed 2): cluster
+    labelling, sizes, ranks and the relabelled field on the GPU per step,
+    inputs resident; the oracle's sweeps on the host beside it on one
+    128^3 octant."""
+    import ctypes as ct
+
+    import mat3d_amd as m
+    from mat3d_amd import gen
+
+    n = 256
+    N = n**3
+    ids = gen.fibres(n, 4, 0.30, seed=2, length=64, device=True)
+    p = m.percolation(ids, n, n, n, 1, 0)
+    L = m.lib()
+    k, vol = ct.c_int(0), ct.c_double(0)
+
+    def run():
+        rc = L.m3d_perc_run(p._h, 1, ct.byref(k), ct.byref(vol))
+        if rc != 0:
+            raise m.M3DError(rc, L.m3d_perc_last_error(p._h).decode())
+
+    for _ in range(max(1, args.warmup)):
+        run()
+    steps = max(args.steps, 10)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    dt = time.perf_counter() - t0
+    value = N * steps / dt
+    # stage durations from events on the handle's stream (separate runs)
+    p.timings(True)
+    st = {}
+    for _ in range(5):
+        run()
+        for key, v in p.timings().items():
+            st[key] = st.get(key, 0.0) + v / 5
+    p.timings(False)
+    # k_link moves at least the flag byte of every cube corner voxel and the
+    # parent (4 B) of every interest voxel
+    link_bytes = N * 1 + 4 * vol.value
+    link_gbs = link_bytes / (st["link"] * 1e-3) / 1e9
+    # the whole analysis: phase ids in (4 B), relabelled field out (4 B)
+    pipe_gbs = 8 * N / (dt / steps) / 1e9
+    print(f"[bench] perc256: {value:.4e} voxels/s, {dt / steps * 1e3:.3f} ms per analysis, {k.value} clusters, "
+          f"volume {vol.value:.0f}; stages (ms) " + ", ".join(f"{a} {b:.3f}" for a, b in st.items()), flush=True)
+    p.runPercolationAnalysis(1)
+    largest = p.clusters_vol_frac[0]
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        h = n // 2
+        sub = np.ascontiguousarray(ids[:h, :h, :h])
+        t1 = time.perf_counter()
+        o = oracle.Percolation(sub, 1, 0).run()
+        tc = time.perf_counter() - t1
+        cpu = {"value": h**3 / tc, "unit": "voxels/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's 5 x 8 x 6 sweeps on the 128^3 octant (0..127)^3 of the same structure "
+                         f"({tc:.1f}s, {len(o[1])} clusters, fixed point {o[4]})"}
+        print(f"[bench] cpu oracle: {h**3 / tc:.4e} voxels/s on 1 thread", flush=True)
+    _line(args, "voxels/s, 256^3 percolation analysis of the fibre phase", value, "voxels/s", dt / steps * 1e3,
+          "int32", "256^3 fibres (radius 4, 30%, seed 2), runPercolationAnalysis of phase 0",
+          {"clusters": k.value, "largest_cluster_fraction": largest, "stages_ms": {a: round(b, 4) for a, b in st.items()},
+           "roofline": {"bound": "hbm", "kernel": "k_link", "achieved": round(link_gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(link_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_launch": link_bytes},
+           "roofline_analysis": {"bytes_per_voxel": 8, "achieved": round(pipe_gbs, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(pipe_gbs / HBM_PEAK_GBS, 4)},
+           "cpu_baseline": cpu})
+
+
 OTHER = {"voronoi_batch": bench_voronoi_batch, "thermal512": bench_thermal512, "j2_192": bench_j2_192,
-         "basic64": bench_basic64, "rec128": bench_rec128, "plate512": bench_plate512}
+         "basic64": bench_basic64, "rec128": bench_rec128, "plate512": bench_plate512, "perc256": bench_perc256}
 
 
 if __name__ == "__main__":

@@ -336,6 +336,53 @@ int m3d_2d_homogenization(m3d_2d *h, double *CHomo, double *epHomo, double *sigH
 /* public epsilon / sigma (3 x Nx*Ny complex) -- .hpp:28-29 */
 int m3d_2d_get_fields(m3d_2d *h, double *eps_ri, double *sig_ri);
 
+/* ---------------------------------------------------------------------
+ * percolation (include/percolation.hpp, src/percolation/percolation.cpp):
+ * clusters of one phase and the face contacts of two phases.  rve: Nx*Ny*Nz
+ * int32 phase ids, index (x*Ny + y)*Nz + z (the reference's C1[k + j*NZ +
+ * i*NY*NZ]).  Clusters: the interest voxels joined through every 2x2x2 cube
+ * (largest corner (x, y, z)) for which one of the six index permutations of
+ * (x, y, z) is an interest voxel -- the fixed point of the reference's
+ * min-label sweeps (:117-286), which its 5 x 8 x 6 sweeps reach on all but
+ * long winding clusters.
+ * ------------------------------------------------------------------- */
+typedef struct m3d_perc m3d_perc;
+
+/* percolation(MatrixDdd &RVE, Nx1, Ny1, Nz1, noOfPhases, intrestPhase) --
+ * percolation.hpp:39, .cpp:7-30.  The field is copied to the GPU. */
+int m3d_perc_create(const int32_t *rve, int nx, int ny, int nz, int noOfPhases, int intrestPhase, m3d_perc **out);
+int m3d_perc_destroy(m3d_perc *h);
+const char *m3d_perc_last_error(m3d_perc *h);
+
+/* runPercolationAnalysis(int total_iter) -- .hpp:42, .cpp:117-382
+ * (total_iter is unused there and here).  Needs Nx = Ny = Nz, and every
+ * phase id below noOfPhases + 1 (M3D_ERR_ARG otherwise: such ids read as
+ * cluster labels in the reference).  nclusters = all_clusters.size();
+ * vol_intrstd = vol_intrstd_phs (interest voxels; the reference adds them up
+ * again on every call, this returns one call's count). */
+int m3d_perc_run(m3d_perc *h, int total_iter, int *nclusters, double *vol_intrstd);
+
+/* C1 after the analysis (interest voxels = noOfPhases + 1 + rank of their
+ * cluster by size, equal sizes equal rank; other voxels their id) or after
+ * SurfaceOfTwoPhase (the field given there); all_clusters -- .hpp:59: the
+ * clusters' labels (noOfPhases + 1 + raster rank of their first voxel among
+ * the interest voxels) in first-appearance order; clusters_vol_frac -- the
+ * clusters' voxel counts / vol_intrstd_phs, descending.  Each may be NULL;
+ * at most maxn clusters are written. */
+int m3d_perc_result(m3d_perc *h, int32_t *c1, int32_t *all_clusters, double *clusters_vol_frac, int maxn);
+
+/* SurfaceOfTwoPhase(MatrixDdd &RVE, phase1, phase2) -- .hpp:47, .cpp:384-414:
+ * voxels of phase1 times their -x, -y and -z face neighbours of phase2 (-1
+ * outside the box); the field replaces the handle's, as it replaces C1.
+ * Counted in 64 bits (the reference's int). */
+int m3d_perc_surface(m3d_perc *h, const int32_t *rve, int phase1, int phase2, long long *commonSurface);
+
+/* No reference counterpart: stage durations (ms) of the next runs, from
+ * events on the handle's stream: init, link, compress, enumerate, sizes,
+ * sort, relabel (7 values). */
+int m3d_perc_set_timing(m3d_perc *h, int on);
+int m3d_perc_timings(m3d_perc *h, double *ms, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ EXPORTS = [
     "m3d_gen_last_error", "m3d_gen_balls", "m3d_gen_cylinders", "m3d_gen_voronoi",
     "m3d_2d_create", "m3d_2d_destroy", "m3d_2d_last_error", "m3d_2d_material", "m3d_2d_set_load", "m3d_2d_run",
     "m3d_2d_homogenization", "m3d_2d_get_fields",
+    "m3d_perc_create", "m3d_perc_destroy", "m3d_perc_last_error", "m3d_perc_run", "m3d_perc_result",
+    "m3d_perc_surface", "m3d_perc_set_timing", "m3d_perc_timings",
 ]
 
 
@@ -115,6 +117,14 @@ def lib():
             "m3d_2d_run": (ip, [vp, ipp]),
             "m3d_2d_homogenization": (ip, [vp, dp, dp, dp]),
             "m3d_2d_get_fields": (ip, [vp, dp, dp]),
+            "m3d_perc_create": (ip, [i32p, ip, ip, ip, ip, ip, ct.POINTER(vp)]),
+            "m3d_perc_destroy": (ip, [vp]),
+            "m3d_perc_last_error": (cp, [vp]),
+            "m3d_perc_run": (ip, [vp, ip, ipp, dp]),
+            "m3d_perc_result": (ip, [vp, i32p, i32p, dp, ip]),
+            "m3d_perc_surface": (ip, [vp, i32p, ip, ip, ct.POINTER(ct.c_longlong)]),
+            "m3d_perc_set_timing": (ip, [vp, ip]),
+            "m3d_perc_timings": (ip, [vp, dp, ip]),
             "m3d_thermal_create": (ip, [ip, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_thermal_destroy": (ip, [vp]),
             "m3d_thermal_material": (ip, [vp, ct.c_double, ct.c_double, i32p]),
@@ -526,3 +536,82 @@ class FFT_Analysis2D:
         s = np.zeros((self.xy, 3), dtype=np.complex128)
         self._chk(lib().m3d_2d_get_fields(self._h, _dp(e.view(np.float64)), _dp(s.view(np.float64))))
         return e, s
+
+
+def _phase_ids(RVE, n):
+    """RVE values as int32 phase ids, truncated toward zero as the
+    reference's int conversion C1[i] = RVE(i) does."""
+    r = np.asarray(RVE).reshape(-1)
+    if r.size != n:
+        raise ValueError(f"RVE has {r.size} values, expected {n}")
+    return np.ascontiguousarray(r.astype(np.int32))
+
+
+class percolation:
+    """percolation(RVE, Nx1, Ny1, Nz1, noOfPhases, intrestPhase)
+    (include/percolation.hpp:36-71, src/percolation/percolation.cpp): the
+    clusters of one phase and the face contacts of two phases, on the GPU.
+    RVE holds Nx*Ny*Nz phase ids, index (x*Ny + y)*Nz + z.  Public results
+    as in the reference: all_clusters, clusters_vol_frac (descending),
+    vol_intrstd_phs, commonSurface, and C1 (the relabelled field, shape
+    (Nx, Ny, Nz))."""
+
+    def __init__(self, RVE, Nx1, Ny1, Nz1, noOfPhases, intrestPhase):
+        self.NX, self.NY, self.NZ = int(Nx1), int(Ny1), int(Nz1)
+        self.noOfPhases, self.intrestPhase = int(noOfPhases), int(intrestPhase)
+        ids = _phase_ids(RVE, self.NX * self.NY * self.NZ)
+        h = ct.c_void_p()
+        rc = lib().m3d_perc_create(_i32p(ids), self.NX, self.NY, self.NZ, self.noOfPhases, self.intrestPhase,
+                                   ct.byref(h))
+        if rc != 0:
+            raise M3DError(rc, "m3d_perc_create failed")
+        self._h = h
+        self.all_clusters = []
+        self.clusters_vol_frac = []
+        self.vol_intrstd_phs = 0.0
+        self.commonSurface = 0
+        self.C1 = ids.reshape(self.NX, self.NY, self.NZ)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise M3DError(rc, lib().m3d_perc_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().m3d_perc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def runPercolationAnalysis(self, total_iter=1):
+        """percolation.cpp:117-382 (total_iter is unused, as there)."""
+        k, vol = ct.c_int(0), ct.c_double(0)
+        self._chk(lib().m3d_perc_run(self._h, int(total_iter), ct.byref(k), ct.byref(vol)))
+        c1 = np.empty((self.NX, self.NY, self.NZ), dtype=np.int32)
+        cl = np.zeros(max(1, k.value), dtype=np.int32)
+        fr = np.zeros(max(1, k.value), dtype=np.float64)
+        self._chk(lib().m3d_perc_result(self._h, _i32p(c1), _i32p(cl), _dp(fr), k.value))
+        self.C1 = c1
+        self.all_clusters = [int(v) for v in cl[: k.value]]
+        self.clusters_vol_frac = [float(v) for v in fr[: k.value]]
+        self.vol_intrstd_phs = vol.value
+
+    def SurfaceOfTwoPhase(self, RVE, phase1, phase2):
+        """percolation.cpp:384-414: sets commonSurface (and C1 to RVE)."""
+        ids = _phase_ids(RVE, self.NX * self.NY * self.NZ)
+        c = ct.c_longlong(0)
+        self._chk(lib().m3d_perc_surface(self._h, _i32p(ids), int(phase1), int(phase2), ct.byref(c)))
+        self.commonSurface = c.value
+        self.C1 = ids.reshape(self.NX, self.NY, self.NZ)
+        return c.value
+
+    def timings(self, on=None):
+        """Stage durations (ms) of the last run; timings(True) turns the
+        events on for the next runs."""
+        if on is not None:
+            self._chk(lib().m3d_perc_set_timing(self._h, int(bool(on))))
+            return None
+        ms = np.zeros(7)
+        self._chk(lib().m3d_perc_timings(self._h, _dp(ms), 7))
+        return dict(zip(("init", "link", "compress", "enumerate", "sizes", "sort", "relabel"), ms.tolist()))

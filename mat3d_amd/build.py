@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", f) for f in ("m3d_elastic.hip", "m3d_thermal.hip", "m3d_basic.hip", "m3d_rec.hip",
-                                                   "m3d_gen.hip", "m3d_2d.hip")]
+                                                   "m3d_gen.hip", "m3d_2d.hip", "m3d_perc.hip")]
 LIB = os.path.join(HERE, "libmat3d.so")
 
 HIPCC_FLAGS = [

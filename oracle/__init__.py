@@ -96,6 +96,12 @@ def lib():
             "m3o_2d_run": (None, [vp]),
             "m3o_2d_homogenization": (None, [vp, dp, dp, dp]),
             "m3o_2d_fields": (None, [vp, dp, dp]),
+            "m3o_perc_create": (vp, [i32p, ip, ip, ip, ip, ip]),
+            "m3o_perc_destroy": (None, [vp]),
+            "m3o_perc_run": (None, [vp]),
+            "m3o_perc_result": (ip, [vp, i32p, ipp, dp, dp, ip]),
+            "m3o_perc_surface": (ip, [vp, i32p, ip, ip]),
+            "m3o_perc_fixed_point": (ip, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -410,3 +416,42 @@ class Elastic2D:
         s = np.zeros((n, 3), dtype=np.complex128)
         lib().m3o_2d_fields(self._h, _dp(e.view(np.float64)), _dp(s.view(np.float64)))
         return e, s
+
+
+class Percolation:
+    """percolation (src/percolation/percolation.cpp): the reference's
+    min-label sweeps (5 rounds x 8 orders x 6 trigger tests), cluster list,
+    volume fractions and relabelling, and SurfaceOfTwoPhase."""
+
+    def __init__(self, rve, nph, phase):
+        r = np.ascontiguousarray(rve, dtype=np.int32)
+        self.shape = r.shape
+        self._r = r.reshape(-1)
+        self._h = lib().m3o_perc_create(_i32p(self._r), *self.shape, int(nph), int(phase))
+
+    def close(self):
+        if self._h:
+            lib().m3o_perc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self):
+        """-> (C1, all_clusters, clusters_vol_frac, vol_intrstd_phs, fixed point reached)"""
+        if len(set(self.shape)) != 1:
+            raise ValueError("the reference's sweeps need Nx = Ny = Nz")
+        lib().m3o_perc_run(self._h)
+        n = int(np.prod(self.shape))
+        c1 = np.empty(n, dtype=np.int32)
+        cl = np.zeros(n + 1, dtype=np.int32)
+        fr = np.zeros(n + 1)
+        vol = ct.c_double(0)
+        k = lib().m3o_perc_result(self._h, _i32p(c1), cl.ctypes.data_as(ct.POINTER(ct.c_int)), _dp(fr),
+                                  ct.byref(vol), n + 1)
+        return (c1.reshape(self.shape), cl[:k].copy(), fr[:k].copy(), vol.value,
+                bool(lib().m3o_perc_fixed_point(self._h)))
+
+    def surface(self, rve, p1, p2):
+        r = np.ascontiguousarray(rve, dtype=np.int32).reshape(-1)
+        return int(lib().m3o_perc_surface(self._h, _i32p(r), int(p1), int(p2)))

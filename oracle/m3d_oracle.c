@@ -1813,3 +1813,223 @@ void m3o_2d_fields(const m3o_2d *s, double *eps_ri, double *sig_ri)
     if (eps_ri) memcpy(eps_ri, s->eps, sizeof(cplx) * (size_t)(3 * s->xy));
     if (sig_ri) memcpy(sig_ri, s->sig, sizeof(cplx) * (size_t)(3 * s->xy));
 }
+
+/* ------------------------------------------------------------------------ */
+/* percolation (src/percolation/percolation.cpp): cluster labelling of one   */
+/* phase by the reference's sweeps, cluster volume fractions, and the count  */
+/* of face contacts between two phases.  Grids are cubic (Nx = Ny = Nz), as  */
+/* the reference's permuted-index sweeps require.                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int NX, NY, NZ, Nx, Ny, Nz;
+    int noOfPhases, intrestPhase, phasePlus;
+    int *C, *C1;
+    double vol;
+    int nclusters;
+    int *all_clusters;       /* first-appearance order */
+    double *vol_frac_sorted; /* descending */
+} m3o_perc;
+
+/* percolation(RVE, Nx1, Ny1, Nz1, noOfPhases, intrestPhase), :7-30 */
+m3o_perc *m3o_perc_create(const int32_t *rve, int NX, int NY, int NZ, int nph, int phase)
+{
+    m3o_perc *p = (m3o_perc *)xcalloc(1, sizeof(m3o_perc));
+    p->NX = NX; p->NY = NY; p->NZ = NZ;
+    p->Nx = NX + 2; p->Ny = NY + 2; p->Nz = NZ + 2;
+    p->noOfPhases = nph; p->intrestPhase = phase; p->phasePlus = nph + 1;
+    const long n = (long)NX * NY * NZ;
+    p->C1 = (int *)xcalloc((size_t)n, sizeof(int));
+    p->C = (int *)xcalloc((size_t)p->Nx * p->Ny * p->Nz, sizeof(int));
+    for (long i = 0; i < n; ++i) p->C1[i] = rve[i];
+    return p;
+}
+
+void m3o_perc_destroy(m3o_perc *p)
+{
+    if (!p) return;
+    free(p->C); free(p->C1); free(p->all_clusters); free(p->vol_frac_sorted);
+    free(p);
+}
+
+#define PC(i, j, k) p->C[(long)(k) + (long)(j) * p->Nz + (long)(i) * p->Ny * p->Nz]
+
+/* setMargin(cd), :36-59 */
+static void perc_setMargin(m3o_perc *p, int cd)
+{
+    int lab = p->phasePlus;
+    for (int i = 0; i < p->Nx; ++i)
+        for (int j = 0; j < p->Ny; ++j)
+            for (int k = 0; k < p->Nz; ++k) {
+                if (i == p->Nx - 1 || j == p->Ny - 1 || k == p->Nz - 1 || i == 0 || j == 0 || k == 0)
+                    PC(i, j, k) = -1;
+                else {
+                    const int v = p->C1[(long)(k - 1) + (long)(j - 1) * p->NZ + (long)(i - 1) * p->NY * p->NZ];
+                    if (v == p->intrestPhase && cd == 0) {
+                        PC(i, j, k) = lab;
+                        lab += 1;
+                        p->vol += 1;
+                    } else
+                        PC(i, j, k) = v;
+                }
+            }
+}
+
+/* neighborCheck(i, j, k), :75-114: the interest voxels of the cube
+ * (i-1..i, j-1..j, k-1..k) all take their minimum label */
+static void perc_neighborCheck(m3o_perc *p, int i, int j, int k)
+{
+    int ii[8], jj[8], kk[8], lab[8], cnt = 0;
+    for (int t = 0; t < 2; ++t)
+        for (int r = 0; r < 2; ++r)
+            for (int w = 0; w < 2; ++w) {
+                const int a = i + (t ? 0 : -1), b = j + (r ? 0 : -1), c = k + (w ? 0 : -1);
+                if (PC(a, b, c) >= p->phasePlus) {
+                    ii[cnt] = a; jj[cnt] = b; kk[cnt] = c; lab[cnt] = PC(a, b, c);
+                    cnt += 1;
+                }
+            }
+    double mn = 1e99;
+    for (int q = 0; q < cnt; ++q) mn = fmin(mn, lab[q] * 1.);
+    for (int q = 0; q < cnt; ++q) PC(ii[q], jj[q], kk[q]) = (int)mn;
+}
+
+static int perc_cond(m3o_perc *p, int t, int i, int j, int k)
+{
+    switch (t) {
+    case 1: return PC(i, j, k) >= p->phasePlus;
+    case 2: return PC(i, k, j) >= p->phasePlus;
+    case 3: return PC(j, i, k) >= p->phasePlus;
+    case 4: return PC(j, k, i) >= p->phasePlus;
+    case 5: return PC(k, j, i) >= p->phasePlus;
+    default: return PC(k, i, j) >= p->phasePlus;
+    }
+}
+
+static int perc_cmp_desc(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x < y) - (x > y);
+}
+
+/* runPercolationAnalysis(total_iter), :117-380 (total_iter is unused there:
+ * five rounds of the eight sweep orders, each over the six index
+ * permutations of the trigger test); then the clusters in first-appearance
+ * order, their volume fractions, and every interest voxel relabelled
+ * phasePlus + rank of its cluster's fraction (equal fractions, equal rank) */
+void m3o_perc_run(m3o_perc *p)
+{
+    perc_setMargin(p, 0);
+    const int lo = 1, hiX = p->Nx - 2, hiY = p->Ny - 2, hiZ = p->Nz - 2;
+    for (int X = 0; X < 5; ++X)
+        for (int s = 0; s < 8; ++s) {
+            const int di = (s & 4) ? -1 : 1, dj = (s & 2) ? -1 : 1, dk = (s & 1) ? -1 : 1;
+            for (int t = 1; t < 7; ++t)
+                for (int i = (di > 0 ? lo : hiX); di > 0 ? i <= hiX : i >= lo; i += di)
+                    for (int j = (dj > 0 ? lo : hiY); dj > 0 ? j <= hiY : j >= lo; j += dj)
+                        for (int k = (dk > 0 ? lo : hiZ); dk > 0 ? k <= hiZ : k >= lo; k += dk)
+                            if (perc_cond(p, t, i, j, k)) perc_neighborCheck(p, i, j, k);
+        }
+    /* removeMargin, :62-70 */
+    for (int i = 0; i < p->NX; ++i)
+        for (int j = 0; j < p->NY; ++j)
+            for (int k = 0; k < p->NZ; ++k)
+                p->C1[(long)k + (long)j * p->NZ + (long)i * p->NY * p->NZ] = PC(i + 1, j + 1, k + 1);
+    const long n = (long)p->NX * p->NY * p->NZ;
+    /* clusters in first-appearance order (:318-336) and their fractions */
+    int cap = 16, nc = 0;
+    int *cl = (int *)malloc(sizeof(int) * (size_t)cap);
+    for (long i = 0; i < n; ++i)
+        if (p->C1[i] >= p->phasePlus) {
+            int f = 0;
+            for (int q = 0; q < nc; ++q)
+                if (cl[q] == p->C1[i]) { f = 1; break; }
+            if (!f) {
+                if (nc == cap) { cap *= 2; cl = (int *)realloc(cl, sizeof(int) * (size_t)cap); }
+                cl[nc++] = p->C1[i];
+            }
+        }
+    double *fr = (double *)xcalloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+    double *srt = (double *)xcalloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+    for (int q = 0; q < nc; ++q) {
+        double s = 0;
+        for (long i = 0; i < n; ++i)
+            if (p->C1[i] == cl[q]) s += 1;
+        fr[q] = s / p->vol;
+        srt[q] = fr[q];
+    }
+    qsort(srt, (size_t)nc, sizeof(double), perc_cmp_desc);
+    /* relabel (:355-376): std::find of the cluster's fraction in the sorted list */
+    for (long i = 0; i < n; ++i)
+        if (p->C1[i] >= p->phasePlus) {
+            int i1 = 0;
+            while (cl[i1] != p->C1[i]) ++i1;
+            int i2 = 0;
+            while (srt[i2] != fr[i1]) ++i2;
+            p->C1[i] = p->phasePlus + i2;
+        }
+    free(fr);
+    p->nclusters = nc;
+    p->all_clusters = cl;
+    p->vol_frac_sorted = srt;
+}
+
+/* out: relabelled phase field; returns the cluster count */
+int m3o_perc_result(const m3o_perc *p, int32_t *c1, int *all_clusters, double *vol_frac, double *vol_intrstd, int maxn)
+{
+    const long n = (long)p->NX * p->NY * p->NZ;
+    if (c1) for (long i = 0; i < n; ++i) c1[i] = p->C1[i];
+    for (int q = 0; q < p->nclusters && q < maxn; ++q) {
+        if (all_clusters) all_clusters[q] = p->all_clusters[q];
+        if (vol_frac) vol_frac[q] = p->vol_frac_sorted[q];
+    }
+    if (vol_intrstd) *vol_intrstd = p->vol;
+    return p->nclusters;
+}
+
+/* SurfaceOfTwoPhase(RVE, phase1, phase2), :384-414: voxels of phase1 times
+ * their -x, -y, -z face neighbours of phase2 (the margin is -1) */
+int m3o_perc_surface(m3o_perc *p, const int32_t *rve, int phase1, int phase2)
+{
+    const long n = (long)p->NX * p->NY * p->NZ;
+    for (long i = 0; i < n; ++i) p->C1[i] = rve[i];
+    perc_setMargin(p, 1);
+    int count = 0;
+    for (int i = 0; i < p->Nx; ++i)
+        for (int j = 0; j < p->Ny; ++j)
+            for (int k = 0; k < p->Nz; ++k)
+                if (i != p->Nx - 1 && j != p->Ny - 1 && k != p->Nz - 1 && i * j * k != 0 && PC(i, j, k) == phase1)
+                    for (int t = 0; t < 2; ++t)
+                        for (int r = 0; r < 2; ++r)
+                            for (int w = 0; w < 2; ++w) {
+                                const int a = i + (t ? 0 : -1), b = j + (r ? 0 : -1), c = k + (w ? 0 : -1);
+                                const int diff = abs(a - i) + abs(b - j) + abs(c - k);
+                                if (diff == 1 && PC(a, b, c) == phase2) count += 1;
+                            }
+    return count;
+}
+#undef PC
+
+/* 1 when the last analysis stopped at a fixed point of its sweeps: every
+ * cube that a trigger test would visit holds one label on its interest
+ * voxels (then a further sweep would change nothing) */
+int m3o_perc_fixed_point(const m3o_perc *p)
+{
+    m3o_perc *q = (m3o_perc *)p;
+    for (int i = 1; i < q->Nx - 1; ++i)
+        for (int j = 1; j < q->Ny - 1; ++j)
+            for (int k = 1; k < q->Nz - 1; ++k) {
+                int act = 0;
+                for (int t = 1; t < 7 && !act; ++t) act = perc_cond(q, t, i, j, k);
+                if (!act) continue;
+                int lab = -2;
+                for (int a = i - 1; a <= i; ++a)
+                    for (int b = j - 1; b <= j; ++b)
+                        for (int c = k - 1; c <= k; ++c) {
+                            const int v = q->C[(long)c + (long)b * q->Nz + (long)a * q->Ny * q->Nz];
+                            if (v < q->phasePlus) continue;
+                            if (lab == -2) lab = v;
+                            else if (v != lab) return 0;
+                        }
+            }
+    return 1;
+}
