@@ -29,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# HBM bytes per launch of the percolation link kernel on the 256^3 fibre
+# structure: FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-calibrated), rocprofv3 --pmc
+MEASURED_TRAFFIC_PERC_LINK = 259.2e6
 
 METRIC = "voxel-iters/s at 256³ (FP32+FP64 polish vs FP64 GPU), C_eff rel err"
 
@@ -549,7 +552,7 @@ def bench_perc256(args):
           "int32", "256^3 fibres (radius 4, 30%, seed 2), runPercolationAnalysis of phase 0",
           {"clusters": k.value, "largest_cluster_fraction": largest, "stages_ms": {a: round(b, 4) for a, b in st.items()},
            "roofline": {"bound": "hbm", "kernel": "k_link", "achieved": round(link_gbs, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(link_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(link_gbs / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_PERC_LINK,
                         "bytes_per_launch": link_bytes},
            "roofline_analysis": {"bytes_per_voxel": 8, "achieved": round(pipe_gbs, 1), "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": round(pipe_gbs / HBM_PEAK_GBS, 4)},
