@@ -3,6 +3,9 @@
 //
 //   FFT_Analysis3D          -- include/FFT_analysis3D.hpp:36-215
 //   FFT_analysis3d_thermal  -- include/FFT_thermal3D.hpp:37-154
+//   recPlan (TPCF form)     -- include/reconstruction.hpp:55-76
+//   FFT_Analysis2D          -- include/FFT_analysis2D.hpp:20-99
+//   percolation             -- include/percolation.hpp:36-71
 //
 // Method names, argument order and meaning are the reference's.  The matrix
 // arguments are templates: anything with data()/rows()/cols() in the
@@ -14,6 +17,8 @@
 // e.g. src/3d/FFT_analysis3D.cpp:672-673).
 #pragma once
 
+#include <complex>
+#include <cstdint>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -248,6 +253,171 @@ class FFT_analysis3d_thermal {
     bool have_load_ = false;
     double km_ = 0, ki_ = 0;
     long long rve_sum_ = 0;
+};
+
+// recPlan(double *TPCF, int Nx1, int Ny1, int Nz1), reconstruction.hpp:55, and
+// runRec(int n, double b), :66.  mTilda holds Nx*Ny*Nz complex values as
+// (re, im) pairs, fftw_complex's layout; trackrealerrorA one error per
+// iteration.  The random phases come from a seeded stream (seed).
+class recPlan {
+  public:
+    recPlan(const double* TPCF, int Nx1, int Ny1, int Nz1, uint64_t seed = 1)
+        : xyz((long long)Nx1 * Ny1 * Nz1), seed_(seed)
+    {
+        int rc = m3d_rec_create(TPCF, Nx1, Ny1, Nz1, &h_);
+        if (rc != M3D_OK) throw Error(rc, "m3d_rec_create failed");
+    }
+    ~recPlan() { m3d_rec_destroy(h_); }
+    recPlan(const recPlan&) = delete;
+    recPlan& operator=(const recPlan&) = delete;
+
+    void runRec(int n, double b)
+    {
+        int it = 0;
+        chk(m3d_rec_run(h_, n, b, seed_, &it, &status, &search_steps));
+        mTilda.assign((size_t)(2 * xyz), 0.0);
+        trackrealerrorA.assign((size_t)(it > 0 ? it : 1), 0.0);
+        int k = 0;
+        chk(m3d_rec_result(h_, mTilda.data(), trackrealerrorA.data(), (int)trackrealerrorA.size(), &k));
+        trackrealerrorA.resize((size_t)k);
+    }
+
+    std::vector<double> mTilda, trackrealerrorA;
+    int status = 0, search_steps = 0;
+    long long xyz;
+
+  private:
+    void chk(int rc)
+    {
+        if (rc != M3D_OK) throw Error(rc, m3d_rec_last_error(h_));
+    }
+    m3d_rec* h_ = nullptr;
+    uint64_t seed_;
+};
+
+// FFT_Analysis2D(int Nx1, int Ny1), FFT_analysis2D.hpp:82.  epsilon / sigma:
+// 3 x Nx*Ny complex values, (re, im) pairs; CHomo, epHomo, sigHomo: 3x3
+// column-major after homogenization().
+class FFT_Analysis2D {
+  public:
+    FFT_Analysis2D(int Nx1, int Ny1) : xy((long long)Nx1 * Ny1)
+    {
+        int rc = m3d_2d_create(Nx1, Ny1, &h_);
+        if (rc != M3D_OK) throw Error(rc, "m3d_2d_create failed");
+    }
+    ~FFT_Analysis2D() { m3d_2d_destroy(h_); }
+    FFT_Analysis2D(const FFT_Analysis2D&) = delete;
+    FFT_Analysis2D& operator=(const FFT_Analysis2D&) = delete;
+
+    // void material(double *properties, MatrixDdd &RVE1), :89
+    template <class Rve>
+    void material(const double* properties, const Rve& RVE1)
+    {
+        chk(m3d_2d_material(h_, properties, RVE1.data()));
+    }
+    // void setLoad(MatrixCdd &E1), :92 (three complex values)
+    template <class V>
+    void setLoad(const V& E1)
+    {
+        const std::complex<double>* e = E1.data();
+        double ri[6];
+        for (int i = 0; i < 3; ++i) {
+            ri[2 * i] = e[i].real();
+            ri[2 * i + 1] = e[i].imag();
+        }
+        chk(m3d_2d_set_load(h_, ri));
+    }
+    // void runAnalysis(), :95
+    void runAnalysis()
+    {
+        chk(m3d_2d_run(h_, &itNo));
+        fetch();
+    }
+    // void homogenization(), :97
+    void homogenization()
+    {
+        CHomo.assign(9, 0.0);
+        epHomo.assign(9, 0.0);
+        sigHomo.assign(9, 0.0);
+        chk(m3d_2d_homogenization(h_, CHomo.data(), epHomo.data(), sigHomo.data()));
+        fetch();
+    }
+
+    std::vector<double> epsilon, sigma, CHomo, epHomo, sigHomo;
+    int itNo = 0;
+    long long xy;
+
+  private:
+    void fetch()
+    {
+        epsilon.resize((size_t)(6 * xy));
+        sigma.resize((size_t)(6 * xy));
+        chk(m3d_2d_get_fields(h_, epsilon.data(), sigma.data()));
+    }
+    void chk(int rc)
+    {
+        if (rc != M3D_OK) throw Error(rc, m3d_2d_last_error(h_));
+    }
+    m3d_2d* h_ = nullptr;
+};
+
+// percolation(MatrixDdd &RVE, int Nx1, int Ny1, int Nz1, int noOfPhases,
+// int intrestPhase), percolation.hpp:39.  RVE: anything with data() over
+// Nx*Ny*Nz values in the reference's order (MatrixDdd included); values are
+// truncated to int as the reference's C1[i] = RVE(i).  C1 is the relabelled
+// field after runPercolationAnalysis (the given field after
+// SurfaceOfTwoPhase).
+class percolation {
+  public:
+    template <class Rve>
+    percolation(const Rve& RVE, int Nx1, int Ny1, int Nz1, int noOfPhases1, int intrestPhase1)
+        : NX(Nx1), NY(Ny1), NZ(Nz1), noOfPhases(noOfPhases1), intrestPhase(intrestPhase1)
+    {
+        ids(RVE);
+        int rc = m3d_perc_create(C1.data(), NX, NY, NZ, noOfPhases, intrestPhase, &h_);
+        if (rc != M3D_OK) throw Error(rc, "m3d_perc_create failed");
+    }
+    ~percolation() { m3d_perc_destroy(h_); }
+    percolation(const percolation&) = delete;
+    percolation& operator=(const percolation&) = delete;
+
+    // void runPercolationAnalysis(int total_iter), :42
+    void runPercolationAnalysis(int total_iter)
+    {
+        int k = 0;
+        chk(m3d_perc_run(h_, total_iter, &k, &vol_intrstd_phs));
+        all_clusters.assign((size_t)k, 0);
+        clusters_vol_frac.assign((size_t)k, 0.0);
+        chk(m3d_perc_result(h_, C1.data(), all_clusters.data(), clusters_vol_frac.data(), k));
+    }
+    // void SurfaceOfTwoPhase(MatrixDdd &RVE, int phase1, int phase2), :47
+    template <class Rve>
+    void SurfaceOfTwoPhase(const Rve& RVE, int phase1, int phase2)
+    {
+        ids(RVE);
+        chk(m3d_perc_surface(h_, C1.data(), phase1, phase2, &commonSurface));
+    }
+
+    int NX, NY, NZ, noOfPhases, intrestPhase;
+    long long commonSurface = 0;
+    std::vector<int> all_clusters;
+    std::vector<double> clusters_vol_frac;
+    double vol_intrstd_phs = 0;
+    std::vector<int32_t> C1;
+
+  private:
+    template <class Rve>
+    void ids(const Rve& RVE)
+    {
+        const size_t n = (size_t)NX * NY * NZ;
+        C1.resize(n);
+        for (size_t i = 0; i < n; ++i) C1[i] = (int32_t)RVE.data()[i];
+    }
+    void chk(int rc)
+    {
+        if (rc != M3D_OK) throw Error(rc, m3d_perc_last_error(h_));
+    }
+    m3d_perc* h_ = nullptr;
 };
 
 } // namespace m3d

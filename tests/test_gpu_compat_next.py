@@ -1,0 +1,71 @@
+"""The C++ compat classes for the rows beside the 3D solvers
+(include/m3d_compat.hpp: percolation, FFT_Analysis2D, recPlan), driven by
+mat3d_amd/examples/next_rows the way the reference's drivers call its
+classes, give the same public results as the Python mirror on the same
+library."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mat3d_amd as m
+from mat3d_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mat3d_amd", "examples", "next_rows")
+
+
+def run(*args):
+    assert os.path.exists(EXE), "build() compiles the example"
+    out = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout.strip().splitlines()
+
+
+def test_percolation_cpp(tmp_path):
+    n = 32
+    ids = gen.spheres(n, 3, 0.25, seed=4)
+    f = tmp_path / "ids.txt"
+    np.savetxt(f, ids.reshape(-1), fmt="%d")
+    lines = run("perc", n, f, 1, 0, 0, 1)
+    p = m.percolation(ids, n, n, n, 1, 0)
+    p.runPercolationAnalysis(1)
+    c1 = p.C1.reshape(-1).astype(np.int64)
+    h = 0
+    for v in c1.tolist():
+        h = (h * 31 + v) % 1000000007
+    assert lines[0] == f"clusters {len(p.all_clusters)}"
+    assert float(lines[1].split()[1]) == p.vol_intrstd_phs
+    assert lines[2] == f"c1 {int(c1.sum())} {h}"
+    assert [int(v) for v in lines[3].split()[1:]] == p.all_clusters
+    assert [float(v) for v in lines[4].split()[1:]] == p.clusters_vol_frac
+    assert int(lines[5].split()[1]) == p.SurfaceOfTwoPhase(ids, 0, 1)
+
+
+def test_plate2d_cpp():
+    n = 64
+    C = np.array([[float(v) for v in l.split()] for l in run("plate2d", n)])
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    rve = np.where((x - n // 2) ** 2 + (y - n // 2) ** 2 <= (n / 4.0) ** 2, 0.0, 1.0)
+    a = m.FFT_Analysis2D(n, n)
+    a.material([68.9e9, 0.35, 400e9, 0.23], rve.reshape(-1))
+    a.setLoad([0, 0, 0])
+    ref = a.homogenization()
+    assert np.array_equal(C, ref)
+
+
+def test_rec_cpp(tmp_path):
+    n = 32
+    rve = (gen.spheres(n, 3, 0.25, seed=3) == 0).astype(float)
+    tpcf = np.abs(np.fft.fftn(rve)) ** 2 / rve.size
+    f = tmp_path / "tpcf.txt"
+    np.savetxt(f, tpcf.reshape(-1), fmt="%.17g")
+    lines = run("rec", n, f, 40, 0.8, 5)
+    g = m.recPlan(np.loadtxt(f).reshape(n, n, n))
+    it = g.runRec(40, 0.8, 5)
+    assert lines[0] == f"iterations {len(g.trackrealerrorA)}"
+    assert float(lines[1].split()[1]) == g.trackrealerrorA[-1]
+    assert float(lines[2].split()[1]) == pytest.approx(g.mTilda.real.mean(), rel=1e-12)
