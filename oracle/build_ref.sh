@@ -11,7 +11,7 @@ set -u
 REF=/root/reference
 OUT="$(cd "$(dirname "$0")" && pwd)/_ref"
 if [ ! -d "$REF" ]; then
-    echo "build_ref: $REF absent (GPU box); nothing to build"
+    echo "build_ref: $REF absent; nothing to build"
     exit 0
 fi
 missing=()
