@@ -378,8 +378,8 @@ int m3d_perc_result(m3d_perc *h, int32_t *c1, int32_t *all_clusters, double *clu
 int m3d_perc_surface(m3d_perc *h, const int32_t *rve, int phase1, int phase2, long long *commonSurface);
 
 /* No reference counterpart: stage durations (ms) of the next runs, from
- * events on the handle's stream: init, link, compress, enumerate, sizes,
- * sort, relabel (7 values). */
+ * events on the handle's stream: init, link, enumerate, sizes, sort,
+ * relabel (6 values). */
 int m3d_perc_set_timing(m3d_perc *h, int on);
 int m3d_perc_timings(m3d_perc *h, double *ms, int n);
 

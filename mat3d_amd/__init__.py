@@ -612,6 +612,6 @@ class percolation:
         if on is not None:
             self._chk(lib().m3d_perc_set_timing(self._h, int(bool(on))))
             return None
-        ms = np.zeros(7)
-        self._chk(lib().m3d_perc_timings(self._h, _dp(ms), 7))
-        return dict(zip(("init", "link", "compress", "enumerate", "sizes", "sort", "relabel"), ms.tolist()))
+        ms = np.zeros(6)
+        self._chk(lib().m3d_perc_timings(self._h, _dp(ms), 6))
+        return dict(zip(("init", "link", "enumerate", "sizes", "sort", "relabel"), ms.tolist()))

@@ -16,12 +16,11 @@
 //   k_link     union-find over the active cubes, skipping joins that other
 //              joins imply; the roots are lowered by atomic min, so every
 //              root is the smallest index of its set
-//   k_compress every interest voxel points at its root
 //   k_count / k_scan_top / k_enum
 //              roots numbered in raster order (the clusters' first-appearance
 //              order, :318-336) with the raster rank of each root among the
 //              interest voxels (the reference's label)
-//   k_sizes    voxels per cluster
+//   k_sizes    every interest voxel points at its root; voxels per cluster
 //   k_hist / scan / k_rank
 //              counting sort of the sizes: rank = number of clusters that are
 //              strictly larger (std::find of the cluster's fraction in the
@@ -268,13 +267,6 @@ __global__ void __launch_bounds__(NT) k_link(const uint8_t* f, int32_t* lab, int
     }
 }
 
-__global__ void __launch_bounds__(NT) k_compress(const uint8_t* f, int32_t* lab, long long n)
-{
-    const long long v = blockIdx.x * (long long)NT + threadIdx.x;
-    if (v >= n || !f[v]) return;
-    lab[v] = find_root(lab, (int)v);
-}
-
 // interest voxels (ci) and roots (cr) per block of BT elements
 __global__ void __launch_bounds__(NT) k_count(const uint8_t* f, const int32_t* lab, long long n, int* ci, int* cr)
 {
@@ -407,12 +399,13 @@ __device__ __forceinline__ void wave_add(int* a, bool has, int idx)
     }
 }
 
-// voxels per cluster: each workgroup counts BT voxels into a small table in
+// voxels per cluster (and every voxel's parent set to its root): each
+// workgroup counts BT voxels into a small table in
 // LDS (open addressing on the cluster index) and adds the table to csize;
 // a cluster that finds no free slot goes to csize directly
 constexpr int HSLOTS = 256;
 
-__global__ void __launch_bounds__(NT) k_sizes(const uint8_t* f, const int32_t* lab, const int32_t* qm, long long n,
+__global__ void __launch_bounds__(NT) k_sizes(const uint8_t* f, int32_t* lab, const int32_t* qm, long long n,
                                               int* csize)
 {
     __shared__ int key[HSLOTS], val[HSLOTS];
@@ -425,7 +418,13 @@ __global__ void __launch_bounds__(NT) k_sizes(const uint8_t* f, const int32_t* l
     for (int r = 0; r < TPB; ++r) {
         const long long e = blockIdx.x * (long long)BT + r * NT + threadIdx.x;
         bool has = e < n && f[e];
-        const int q = has ? qm[lab[e]] : 0;
+        int q = 0;
+        if (has) {
+            // the voxel's root; stored as its parent for k_relabel
+            const int r = find_root(lab, (int)e);
+            lab[e] = r;
+            q = qm[r];
+        }
         unsigned long long act = __ballot(has);
         while (act) {
             const int leader = __ffsll((long long)act) - 1;
@@ -563,7 +562,7 @@ void dgrow(P*& p, size_t& cap, size_t count)
 }
 
 // stage boundaries timed on the handle's stream
-enum Stage { ST_INIT, ST_LINK, ST_COMPRESS, ST_ENUM, ST_SIZES, ST_SORT, ST_RELABEL, ST_N };
+enum Stage { ST_INIT, ST_LINK, ST_ENUM, ST_SIZES, ST_SORT, ST_RELABEL, ST_N };
 
 } // namespace perc
 } // namespace m3d
@@ -652,9 +651,6 @@ void label(m3d_perc* h)
     mark(h, ST_LINK);
     hipLaunchKernelGGL(k_link, dim3(gb), dim3(NT), 0, s, (const uint8_t*)h->flag, h->lab, h->nx);
     HCHK(hipGetLastError());
-    mark(h, ST_COMPRESS);
-    hipLaunchKernelGGL(k_compress, dim3(gb), dim3(NT), 0, s, (const uint8_t*)h->flag, h->lab, n);
-    HCHK(hipGetLastError());
     mark(h, ST_ENUM);
     hipLaunchKernelGGL(k_count, dim3(cb), dim3(NT), 0, s, (const uint8_t*)h->flag, (const int32_t*)h->lab, n, h->ci, h->cr);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(NTOP), 0, s, h->ci, h->cr, (int)cb, h->tot);
@@ -695,8 +691,8 @@ void stats(m3d_perc* h)
     mark(h, ST_SIZES);
     if (K > 0) {
         hipLaunchKernelGGL(k_fill_i, dim3(std::min(blocks(K, NT), 1024u)), dim3(NT), 0, s, h->csize, (long long)K, 0);
-        hipLaunchKernelGGL(k_sizes, dim3(blocks(n, BT)), dim3(NT), 0, s, (const uint8_t*)h->flag,
-                           (const int32_t*)h->lab, (const int32_t*)h->qm, n, h->csize);
+        hipLaunchKernelGGL(k_sizes, dim3(blocks(n, BT)), dim3(NT), 0, s, (const uint8_t*)h->flag, h->lab,
+                           (const int32_t*)h->qm, n, h->csize);
     }
     HCHK(hipGetLastError());
     mark(h, ST_SORT);
