@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # HBM bytes per launch of the percolation link kernel on the 256^3 fibre
 # structure: FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-calibrated), rocprofv3 --pmc
-MEASURED_TRAFFIC_PERC_LINK = 259.2e6
+MEASURED_TRAFFIC_PERC_LINK = 256.1e6
 
 METRIC = "voxel-iters/s at 256³ (FP32+FP64 polish vs FP64 GPU), C_eff rel err"
 
