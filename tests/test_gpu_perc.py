@@ -168,3 +168,13 @@ def test_fibres_256_properties():
             assert r == 0 or sizes[r - 1] > sizes[r]
             assert counts[r] == sizes[sizes == sizes[r]].sum()
         assert np.array_equal(p.C1[ids != ph], ids[ids != ph])
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 65])
+def test_small_and_odd_boxes(N):
+    rng = np.random.default_rng(N)
+    for d in (0.0, 0.5, 1.0):
+        ids = (rng.random((N, N, N)) < d).astype(np.int32) + 1
+        p, o = both(ids, 2, 2)
+        same(p, o)
+        assert p.SurfaceOfTwoPhase(ids, 2, 1) == surface(ids, 2, 1)
