@@ -398,6 +398,20 @@ class percolation {
         chk(m3d_perc_surface(h_, C1.data(), phase1, phase2, &commonSurface));
     }
 
+    // void printer(string fold, string clustresSizedata, string largestCluster), :53
+    void printer(const std::string& fold, const std::string& clustresSizedata = "", const std::string& largestCluster = "")
+    {
+        (void)clustresSizedata;
+        (void)largestCluster;
+        io::printer_percolation(fold, (long long)C1.size(), C1.data(), intrestPhase, noOfPhases, clusters_vol_frac);
+    }
+    // void vtiWriter(string fileName), :58 -- C1 as one Float64 component
+    void vtiWriter(const std::string& fileName)
+    {
+        std::vector<double> d(C1.begin(), C1.end());
+        io::write_vti(fileName, NX, NY, NZ, 1, {{d.data(), 1}});
+    }
+
     int NX, NY, NZ, noOfPhases, intrestPhase;
     long long commonSurface = 0;
     std::vector<int> all_clusters;

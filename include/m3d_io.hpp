@@ -11,7 +11,9 @@
 //   printer                             src/3d/FFT_analysis3D.cpp:1001-1053,
 //                                       src/3dTherm/FFT_thermal3D.cpp:626-691
 //   write_vti                           src/3d/FFT_analysis3D.cpp:1056-1104,
-//                                       src/3dTherm/FFT_thermal3D.cpp:696-740
+//                                       src/3dTherm/FFT_thermal3D.cpp:696-740,
+//                                       src/percolation/percolation.cpp:443-473
+//   printer_percolation                 src/percolation/percolation.cpp:417-441
 //
 // Text output goes through iostreams with the stream's default formatting,
 // and matrices are printed the way Eigen's default stream operator prints
@@ -393,6 +395,30 @@ inline void printer_thermal(const std::string& folder, long long N, const double
         std::ofstream g(folder + "/tempGradient" + std::to_string(c + 1) + ".txt");
         for (long long i = 0; i < N; ++i) g << grad[c + 3 * i] << std::endl;
     }
+}
+
+// percolation::printer, percolation.cpp:417-441: fold/clustersFile.txt (C1,
+// one value per line) and fold/clustersdata.txt (a header with the interest
+// phase and the number of phases, then one row per cluster: its label
+// phasePlus + c and clusters_vol_frac[c], largest first).  The reference
+// also echoes `fold` to stdout and takes two file names it never uses.
+inline void printer_percolation(const std::string& fold, long long N, const int32_t* C1, int intrestPhase,
+                                int noOfPhases, const std::vector<double>& clusters_vol_frac)
+{
+    mkdir(fold.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    {
+        std::ofstream op0(fold + "/" + "clustersFile.txt");
+        for (long long i = 0; i < N; ++i) op0 << C1[i] << std::endl;
+    }
+    std::ofstream op01(fold + "/" + "clustersdata.txt");
+    op01 << "           intrested phase :  " << intrestPhase << "     Number of the phases: " << noOfPhases << std::endl;
+    op01 << "-------------------------------------------------------------------------" << std::endl;
+    op01 << "connected cluster name" << "        " << "volume fraction custer/volume fraction intrested phase"
+         << std::endl;
+    const int phasePlus = noOfPhases + 1;
+    for (size_t c = 0; c < clusters_vol_frac.size(); ++c)
+        op01 << "        " << (int)c + phasePlus << "                                   " << clusters_vol_frac[c]
+             << std::endl;
 }
 
 // --------------------------------------------------------------------- vti

@@ -2,7 +2,7 @@
 // percolation, FFT_Analysis2D and recPlan (include/m3d_compat.hpp), written
 // the way the reference's own drivers call its classes.
 //
-//   next_rows perc N ids.txt noOfPhases intrestPhase phase1 phase2
+//   next_rows perc N ids.txt noOfPhases intrestPhase phase1 phase2 [folder]
 //   next_rows plate2d N
 //   next_rows rec N tpcf.txt iterations beta seed
 //
@@ -42,6 +42,11 @@ int perc(int argc, char** argv)
     for (int v : p.all_clusters) std::printf(" %d", v);
     std::printf("\nfractions");
     for (double v : p.clusters_vol_frac) std::printf(" %.17g", v);
+    if (argc > 8) {
+        // the reference's file output of the relabelled field
+        p.printer(argv[8], "", "");
+        p.vtiWriter(std::string(argv[8]) + "/clusters.vti");
+    }
     p.SurfaceOfTwoPhase(rve, std::atoi(argv[6]), std::atoi(argv[7]));
     std::printf("\nsurface %lld\n", p.commonSurface);
     return 0;

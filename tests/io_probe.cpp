@@ -58,6 +58,11 @@ int main(int argc, char** argv)
         std::vector<double> a(24 * 2), b(24);
         for (int i = 0; i < 24; ++i) { a[2 * i] = i; a[2 * i + 1] = -i; b[i] = 0.5 * i; }
         write_vti(argv[2], 2, 3, 4, 4, {{a.data(), 2}, {b.data(), 1}});
+    } else if (mode == "perc") {
+        // percolation printer: C1 = 0..23, fractions 0.5, 0.25, 1/3 (phases 2, interest 1)
+        std::vector<int32_t> c1(24);
+        for (int i = 0; i < 24; ++i) c1[i] = i;
+        printer_percolation(argv[2], 24, c1.data(), 1, 2, {0.5, 0.25, 1.0 / 3.0});
     } else {
         return 2;
     }

@@ -30,7 +30,8 @@ def test_percolation_cpp(tmp_path):
     ids = gen.spheres(n, 3, 0.25, seed=4)
     f = tmp_path / "ids.txt"
     np.savetxt(f, ids.reshape(-1), fmt="%d")
-    lines = run("perc", n, f, 1, 0, 0, 1)
+    out = tmp_path / "perc_out"
+    lines = run("perc", n, f, 1, 0, 0, 1, out)
     p = m.percolation(ids, n, n, n, 1, 0)
     p.runPercolationAnalysis(1)
     c1 = p.C1.reshape(-1).astype(np.int64)
@@ -43,6 +44,16 @@ def test_percolation_cpp(tmp_path):
     assert [int(v) for v in lines[3].split()[1:]] == p.all_clusters
     assert [float(v) for v in lines[4].split()[1:]] == p.clusters_vol_frac
     assert int(lines[5].split()[1]) == p.SurfaceOfTwoPhase(ids, 0, 1)
+    # printer / vtiWriter (percolation.cpp:417-473)
+    assert [int(v) for v in open(out / "clustersFile.txt").read().split()] == c1.tolist()
+    rows = open(out / "clustersdata.txt").read().split("\n")
+    assert rows[0] == "           intrested phase :  0     Number of the phases: 1"
+    assert rows[2].startswith("connected cluster name        volume fraction")
+    for c, fr in enumerate(p.clusters_vol_frac):
+        assert rows[3 + c] == "        " + str(2 + c) + " " * 35 + f"{fr:g}"
+    vti = open(out / "clusters.vti").read()
+    vals = vti.split('format="ascii">\n')[1].split("        </DataArray>")[0].split()
+    assert [int(float(v)) for v in vals] == c1.tolist()
 
 
 def test_plate2d_cpp():

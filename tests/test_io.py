@@ -127,3 +127,17 @@ def test_vti_is_valid_image_data(probe, tmp_path):
     i = np.arange(24)
     assert np.array_equal(vals[:, 0], i) and np.array_equal(vals[:, 1], -i)
     assert np.array_equal(vals[:, 2], 0.5 * i) and np.array_equal(vals[:, 3], 0 * i)
+
+
+def test_percolation_printer_layout(probe, tmp_path):
+    """percolation::printer (src/percolation/percolation.cpp:417-441)."""
+    d = tmp_path / "clusters"
+    run(probe, "perc", str(d))
+    assert open(d / "clustersFile.txt").read() == "".join(f"{i}\n" for i in range(24))
+    assert open(d / "clustersdata.txt").read() == (
+        "           intrested phase :  1     Number of the phases: 2\n"
+        "-------------------------------------------------------------------------\n"
+        "connected cluster name        volume fraction custer/volume fraction intrested phase\n"
+        "        3                                   0.5\n"
+        "        4                                   0.25\n"
+        "        5                                   0.333333\n")
