@@ -262,7 +262,7 @@ class FFT_analysis3d_thermal {
 class recPlan {
   public:
     recPlan(const double* TPCF, int Nx1, int Ny1, int Nz1, uint64_t seed = 1)
-        : xyz((long long)Nx1 * Ny1 * Nz1), seed_(seed)
+        : xyz((long long)Nx1 * Ny1 * Nz1), Nx_(Nx1), Ny_(Ny1), Nz_(Nz1), seed_(seed)
     {
         int rc = m3d_rec_create(TPCF, Nx1, Ny1, Nz1, &h_);
         if (rc != M3D_OK) throw Error(rc, "m3d_rec_create failed");
@@ -273,6 +273,7 @@ class recPlan {
 
     void runRec(int n, double b)
     {
+        n_ = n;
         int it = 0;
         chk(m3d_rec_run(h_, n, b, seed_, &it, &status, &search_steps));
         mTilda.assign((size_t)(2 * xyz), 0.0);
@@ -280,6 +281,24 @@ class recPlan {
         int k = 0;
         chk(m3d_rec_result(h_, mTilda.data(), trackrealerrorA.data(), (int)trackrealerrorA.size(), &k));
         trackrealerrorA.resize((size_t)k);
+    }
+
+    // void saveRecImages(string locFldr), :70 -- the text files (rve_rec.txt,
+    // err.txt); the PNG slices need CImg and are not written
+    void saveRecImages(const std::string& locFldr)
+    {
+        io::save_rec_text(locFldr, xyz, mTilda.data(), trackrealerrorA.data(), (int)trackrealerrorA.size(), n_);
+    }
+    // void vtiWriter(string fileName, double* tpcf), :72 -- real part of
+    // mTilda and tpcf as two Float64 components
+    void vtiWriter(const std::string& fileName, const double* tpcf)
+    {
+        std::vector<double> a((size_t)(2 * xyz));
+        for (long long i = 0; i < xyz; ++i) {
+            a[(size_t)(2 * i)] = mTilda[(size_t)(2 * i)];
+            a[(size_t)(2 * i + 1)] = tpcf[i];
+        }
+        io::write_vti(fileName, Nx_, Ny_, Nz_, 2, {{a.data(), 2}});
     }
 
     std::vector<double> mTilda, trackrealerrorA;
@@ -292,6 +311,7 @@ class recPlan {
         if (rc != M3D_OK) throw Error(rc, m3d_rec_last_error(h_));
     }
     m3d_rec* h_ = nullptr;
+    int Nx_, Ny_, Nz_, n_ = 0;
     uint64_t seed_;
 };
 
@@ -342,6 +362,9 @@ class FFT_Analysis2D {
         chk(m3d_2d_homogenization(h_, CHomo.data(), epHomo.data(), sigHomo.data()));
         fetch();
     }
+
+    // void printer(), :103 -- epsilon{1,2,3}.txt and sigma{1,2,3}.txt
+    void printer() { io::printer_2d(xy, epsilon.data(), sigma.data()); }
 
     std::vector<double> epsilon, sigma, CHomo, epHomo, sigHomo;
     int itNo = 0;

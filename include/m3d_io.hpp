@@ -14,6 +14,8 @@
 //                                       src/3dTherm/FFT_thermal3D.cpp:696-740,
 //                                       src/percolation/percolation.cpp:443-473
 //   printer_percolation                 src/percolation/percolation.cpp:417-441
+//   printer_2d                          src/2d/FFT_analysis2D.cpp:328-360
+//   save_rec_text                       src/rec/reconstruction.cpp:409-437 (text part)
 //
 // Text output goes through iostreams with the stream's default formatting,
 // and matrices are printed the way Eigen's default stream operator prints
@@ -419,6 +421,47 @@ inline void printer_percolation(const std::string& fold, long long N, const int3
     for (size_t c = 0; c < clusters_vol_frac.size(); ++c)
         op01 << "        " << (int)c + phasePlus << "                                   " << clusters_vol_frac[c]
              << std::endl;
+}
+
+// FFT_Analysis2D::printer, FFT_analysis2D.cpp:328-360: a "Results" folder
+// (left empty, as there) and epsilon{1,2,3}.txt / sigma{1,2,3}.txt in the
+// current directory, the real parts one per line.  eps_ri, sig_ri: 3 x xy
+// complex values as (re, im) pairs.  The reference writes epsilon into the
+// sigma files too; these hold sigma.
+inline void printer_2d(long long xy, const double* eps_ri, const double* sig_ri)
+{
+    mkdir("Results", S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    for (int c = 0; c < 3; ++c) {
+        std::ofstream e("epsilon" + std::to_string(c + 1) + ".txt");
+        for (long long i = 0; i < xy; ++i) e << eps_ri[2 * (c + 3 * i)] << std::endl;
+    }
+    for (int c = 0; c < 3; ++c) {
+        std::ofstream g("sigma" + std::to_string(c + 1) + ".txt");
+        for (long long i = 0; i < xy; ++i) g << sig_ri[2 * (c + 3 * i)] << std::endl;
+    }
+}
+
+// recPlan::saveRecImages, reconstruction.cpp:409-437, without the PNG slices
+// (CImg): locFldr/rve_rec.txt (real part of mTilda, one per line) and
+// locFldr/err.txt ("voloum fraction= " mean, then the first n - 1 errors of
+// a run of n iterations; a run that stopped early has fewer, and the
+// reference prints its unset entries).
+inline void save_rec_text(const std::string& locFldr, long long xyz, const double* mTilda_ri, const double* err,
+                          int nerr, int n)
+{
+    mkdir(locFldr.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    double s = 0;
+    {
+        std::ofstream op2(locFldr + "/" + "rve_rec.txt");
+        for (long long i = 0; i < xyz; ++i) {
+            op2 << mTilda_ri[2 * i] << std::endl;
+            s += mTilda_ri[2 * i];
+        }
+    }
+    s = s / xyz;
+    std::ofstream op1(locFldr + "/" + "err.txt");
+    op1 << "voloum fraction= " << s << std::endl;
+    for (int i = 0; i < n - 1 && i < nerr; ++i) op1 << err[i] << std::endl;
 }
 
 // --------------------------------------------------------------------- vti

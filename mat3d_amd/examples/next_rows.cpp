@@ -3,8 +3,8 @@
 // the way the reference's own drivers call its classes.
 //
 //   next_rows perc N ids.txt noOfPhases intrestPhase phase1 phase2 [folder]
-//   next_rows plate2d N
-//   next_rows rec N tpcf.txt iterations beta seed
+//   next_rows plate2d N [print]
+//   next_rows rec N tpcf.txt iterations beta seed [folder]
 //
 // Prints the public results one item per line for comparison.
 #include <cstdio>
@@ -69,6 +69,7 @@ int plate2d(int argc, char** argv)
     std::vector<std::complex<double>> E(3);
     a.setLoad(E);
     a.homogenization();
+    if (argc > 3) a.printer(); // epsilon*.txt, sigma*.txt in the current directory
     for (int r = 0; r < 3; ++r)
         std::printf("%.17g %.17g %.17g\n", a.CHomo[r], a.CHomo[r + 3], a.CHomo[r + 6]);
     return 0;
@@ -81,6 +82,10 @@ int rec(int argc, char** argv)
     const std::vector<double> t = read_values(argv[3], (size_t)N * N * N);
     m3d::recPlan r(t.data(), N, N, N, std::strtoull(argv[6], nullptr, 10));
     r.runRec(std::atoi(argv[4]), std::atof(argv[5]));
+    if (argc > 7) {
+        r.saveRecImages(argv[7]);
+        r.vtiWriter(std::string(argv[7]) + "/rec.vti", t.data());
+    }
     double vf = 0;
     for (long long i = 0; i < r.xyz; ++i) vf += r.mTilda[2 * i];
     std::printf("iterations %zu\nerror %.17g\nfraction %.17g\nstatus %d\n", r.trackrealerrorA.size(),

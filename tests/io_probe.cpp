@@ -58,6 +58,20 @@ int main(int argc, char** argv)
         std::vector<double> a(24 * 2), b(24);
         for (int i = 0; i < 24; ++i) { a[2 * i] = i; a[2 * i + 1] = -i; b[i] = 0.5 * i; }
         write_vti(argv[2], 2, 3, 4, 4, {{a.data(), 2}, {b.data(), 1}});
+    } else if (mode == "p2d") {
+        // 2D printer in the current directory: 4 points, eps_c(i) = 10c + i + 0.5, sig = -eps
+        std::vector<double> e(24), g(24);
+        for (int i = 0; i < 4; ++i)
+            for (int c = 0; c < 3; ++c) {
+                e[2 * (c + 3 * i)] = 10 * c + i + 0.5;
+                e[2 * (c + 3 * i) + 1] = 7;
+                g[2 * (c + 3 * i)] = -(10 * c + i + 0.5);
+            }
+        printer_2d(4, e.data(), g.data());
+    } else if (mode == "rec") {
+        // mTilda real 0, 1, 0, 1 (mean 0.5), errors 3, 2, 1 of a 3-iteration run
+        std::vector<double> m = {0, 9, 1, 9, 0, 9, 1, 9}, err = {3, 2, 1};
+        save_rec_text(argv[2], 4, m.data(), err.data(), 3, 3);
     } else if (mode == "perc") {
         // percolation printer: C1 = 0..23, fractions 0.5, 0.25, 1/3 (phases 2, interest 1)
         std::vector<int32_t> c1(24);

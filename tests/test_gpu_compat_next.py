@@ -18,9 +18,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "mat3d_amd", "examples", "next_rows")
 
 
-def run(*args):
+def run(*args, cwd=None):
     assert os.path.exists(EXE), "build() compiles the example"
-    out = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=120, cwd=cwd)
     assert out.returncode == 0, out.stderr
     return out.stdout.strip().splitlines()
 
@@ -56,9 +56,9 @@ def test_percolation_cpp(tmp_path):
     assert [int(float(v)) for v in vals] == c1.tolist()
 
 
-def test_plate2d_cpp():
+def test_plate2d_cpp(tmp_path):
     n = 64
-    C = np.array([[float(v) for v in l.split()] for l in run("plate2d", n)])
+    C = np.array([[float(v) for v in l.split()] for l in run("plate2d", n, "print", cwd=tmp_path)])
     x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
     rve = np.where((x - n // 2) ** 2 + (y - n // 2) ** 2 <= (n / 4.0) ** 2, 0.0, 1.0)
     a = m.FFT_Analysis2D(n, n)
@@ -66,6 +66,13 @@ def test_plate2d_cpp():
     a.setLoad([0, 0, 0])
     ref = a.homogenization()
     assert np.array_equal(C, ref)
+    # printer (FFT_analysis2D.cpp:328-360): real parts of the last load case's fields
+    e, s = a.fields()
+    for c in range(3):
+        got = np.array(open(tmp_path / f"epsilon{c + 1}.txt").read().split(), dtype=float)
+        assert np.allclose(got, e[:, c].real, rtol=1e-5, atol=1e-12 * np.abs(e).max())
+        got = np.array(open(tmp_path / f"sigma{c + 1}.txt").read().split(), dtype=float)
+        assert np.allclose(got, s[:, c].real, rtol=1e-5, atol=1e-12 * np.abs(s).max())
 
 
 def test_rec_cpp(tmp_path):
@@ -74,9 +81,18 @@ def test_rec_cpp(tmp_path):
     tpcf = np.abs(np.fft.fftn(rve)) ** 2 / rve.size
     f = tmp_path / "tpcf.txt"
     np.savetxt(f, tpcf.reshape(-1), fmt="%.17g")
-    lines = run("rec", n, f, 40, 0.8, 5)
+    out = tmp_path / "rec_out"
+    lines = run("rec", n, f, 40, 0.8, 5, out)
     g = m.recPlan(np.loadtxt(f).reshape(n, n, n))
     it = g.runRec(40, 0.8, 5)
     assert lines[0] == f"iterations {len(g.trackrealerrorA)}"
     assert float(lines[1].split()[1]) == g.trackrealerrorA[-1]
     assert float(lines[2].split()[1]) == pytest.approx(g.mTilda.real.mean(), rel=1e-12)
+    # saveRecImages text files and vtiWriter (reconstruction.cpp:409-477)
+    rr = np.array(open(out / "rve_rec.txt").read().split(), dtype=float)
+    assert np.allclose(rr, g.mTilda.real.reshape(-1), rtol=1e-5, atol=1e-6)
+    err = open(out / "err.txt").read().split("\n")
+    assert err[0].startswith("voloum fraction= ") and len([x for x in err[1:] if x]) == min(39, it)
+    vti = open(out / "rec.vti").read()
+    vals = np.array(vti.split('format="ascii">\n')[1].split("        </DataArray>")[0].split(), dtype=float)
+    assert np.allclose(vals.reshape(-1, 2)[:, 1], np.loadtxt(f))

@@ -141,3 +141,21 @@ def test_percolation_printer_layout(probe, tmp_path):
         "        3                                   0.5\n"
         "        4                                   0.25\n"
         "        5                                   0.333333\n")
+
+
+def test_2d_printer_layout(probe, tmp_path):
+    """FFT_Analysis2D::printer (src/2d/FFT_analysis2D.cpp:328-360), real parts
+    one per line; the sigma files hold sigma (the reference writes epsilon)."""
+    subprocess.run([probe, "p2d"], cwd=tmp_path, check=True)
+    assert (tmp_path / "Results").is_dir()
+    for c in range(3):
+        assert open(tmp_path / f"epsilon{c + 1}.txt").read() == "".join(f"{10 * c + i + 0.5:g}\n" for i in range(4))
+        assert open(tmp_path / f"sigma{c + 1}.txt").read() == "".join(f"{-(10 * c + i + 0.5):g}\n" for i in range(4))
+
+
+def test_rec_text_layout(probe, tmp_path):
+    """recPlan::saveRecImages text files (src/rec/reconstruction.cpp:422-436)."""
+    d = tmp_path / "rec"
+    run(probe, "rec", str(d))
+    assert open(d / "rve_rec.txt").read() == "0\n1\n0\n1\n"
+    assert open(d / "err.txt").read() == "voloum fraction= 0.5\n3\n2\n"
