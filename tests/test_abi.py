@@ -69,3 +69,23 @@ def test_batch_host_side_validation():
     L = m.lib()
     assert L.m3d_batch_destroy(None) == -1
     assert L.m3d_batch_create(5, ct.byref(ct.c_void_p())) == -1
+
+
+def test_null_handles_of_the_other_rows():
+    """reconstruction, 2D solver, percolation and generators: null handles and
+    bad sizes come back as codes before any device call."""
+    L = m.lib()
+    out = ct.c_void_p()
+    assert L.m3d_rec_destroy(None) == -1
+    assert L.m3d_rec_last_error(None) is not None
+    assert L.m3d_2d_destroy(None) == -1
+    assert L.m3d_2d_create(0, 8, ct.byref(out)) == -1
+    assert L.m3d_perc_destroy(None) == -1
+    assert L.m3d_perc_last_error(None) == b"null handle"
+    assert L.m3d_perc_run(None, 1, None, None) == -1
+    assert L.m3d_perc_result(None, None, None, None, 0) == -1
+    assert L.m3d_perc_set_timing(None, 1) == -1
+    assert L.m3d_perc_create(None, 8, 8, 8, 2, 1, ct.byref(out)) == -1
+    assert L.m3d_perc_create(None, 0, 8, 8, 2, 1, ct.byref(out)) == -1
+    assert L.m3d_gen_balls(0, 8, 8, None, 0, 1, None) == -1
+    assert L.m3d_gen_voronoi(8, 8, 8, None, 0, None) == -1
