@@ -333,13 +333,17 @@ struct Plan {
         }
     }
 
-    void iteration()
+    // with_x: the re-randomisation path (field and its FFT, gated on the
+    // device); chunks without a re-randomisation iteration leave it out
+    void iteration(bool with_x)
     {
         const int nb = nblk;
         hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, st, state);
         fft3(&state->ctl, mT, MT, -1);
-        hipLaunchKernelGGL(k_rand, dim3(nb), dim3(NT), 0, st, state, X, N);
-        fft3(&state->xctl, X, X, -1);
+        if (with_x) {
+            hipLaunchKernelGGL(k_rand, dim3(nb), dim3(NT), 0, st, state, X, N);
+            fft3(&state->xctl, X, X, -1);
+        }
         hipLaunchKernelGGL(k_phase, dim3(nb), dim3(NT), 0, st, state, MT, X, normC, MH, part, N);
         hipLaunchKernelGGL(k_err, dim3(1), dim3(NT), 0, st, state, part, nb, errd, 1.0 / (double)N);
         fft3(&state->ctl, MH, MH, 1);
@@ -373,26 +377,33 @@ struct Plan {
         HCHK(hipMemcpyAsync(state, &h, sizeof h, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_init, dim3(nblk), dim3(NT), 0, st, state, mT, m0, N);
         HCHK(hipGetLastError());
-        // chunks of iterations in a captured graph, replayed until done
+        // chunks of iterations in captured graphs, replayed until done: one
+        // with the re-randomisation path, one without, chosen per chunk from
+        // the iteration counter (it advances by one per iteration until done)
         const int ck = 8;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < ck; ++k) iteration();
-        HCHK(hipStreamEndCapture(st, &graph));
         struct Guard {
-            hipGraph_t g;
-            hipGraphExec_t* x;
+            hipGraph_t g[2] = {nullptr, nullptr};
+            hipGraphExec_t x[2] = {nullptr, nullptr};
             ~Guard()
             {
-                if (*x) (void)hipGraphExecDestroy(*x);
-                if (g) (void)hipGraphDestroy(g);
+                for (int k = 0; k < 2; ++k) {
+                    if (x[k]) (void)hipGraphExecDestroy(x[k]);
+                    if (g[k]) (void)hipGraphDestroy(g[k]);
+                }
             }
-        } guard{graph, &exec};
-        HCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        } gr;
+        for (int v = 0; v < 2; ++v) {
+            HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < ck; ++k) iteration(v == 1);
+            HCHK(hipStreamEndCapture(st, &gr.g[v]));
+            HCHK(hipGraphInstantiate(&gr.x[v], gr.g[v], nullptr, nullptr, 0));
+        }
         const long long max_replays = (long long)n / ck + 2;
         for (long long r = 0; r <= max_replays; ++r) {
-            HCHK(hipGraphLaunch(exec, st));
+            // iteration i re-randomises when (i + 1) % 150 == 0 (k_begin)
+            bool rr = false;
+            for (int k = 0; k < ck; ++k) rr = rr || (h.i + k + 1) % 150 == 0;
+            HCHK(hipGraphLaunch(gr.x[rr ? 1 : 0], st));
             HCHK(hipMemcpyAsync(&h, state, sizeof h, hipMemcpyDeviceToHost, st));
             HCHK(hipStreamSynchronize(st));
             if (h.ctl.done) break;
