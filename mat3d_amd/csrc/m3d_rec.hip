@@ -4,8 +4,9 @@
 //
 // Per iteration (one captured sequence, replayed in chunks; every kernel
 // exits once the device-side done flag is set):
-//   begin   : j += 1; itNo >= n -> done = 2 (":Iteration Number Exceeded");
-//             every 150th iteration selects a re-randomisation stream
+//   begin   : the previous iteration's end (below); j += 1; itNo >= n ->
+//             done = 2 (":Iteration Number Exceeded"); every 150th iteration
+//             selects a re-randomisation stream
 //   fwd FFT : MT = FFT3(mT)  (z pass out of place, then y and x passes)
 //   [rand]  : X = FFT3(uniform[-1,1) field)   (re-randomisation iterations only)
 //   phase   : err partials sum |(|MT|) - normC| (funcNormC); ang = arg(MT), or
@@ -14,7 +15,8 @@
 //   inv FFT : MH = IFFT3(MH) (unnormalised)
 //   project : mH = MH / xyz; j < 100: the reference's relaxed projection with
 //             beta, else clamp to [0, 1]; mT = mH
-//   end     : j > 200 -> j = 0 (after a clamp iteration); itNo += 1
+//   end     : j > 200 -> j = 0 (after a clamp iteration); itNo += 1 --
+//             marked by the projection, applied by the next begin
 // The 3D complex FFT is z-lines in LDS (contiguous, tiles of TY lines), then
 // the y and x passes of m3d_fast.hpp on a one-component view of the grid.
 // The final volume-fraction search (:374-402) counts voxels above a moving
@@ -53,6 +55,7 @@ struct State {
     Ctl ctl;  // ctl.done: 1 converged, 2 iteration number exceeded
     Ctl xctl; // done unless this iteration re-randomises (gates the FFT of X)
     int i, j, n, rr;
+    int pend; // the last iteration's end (j reset, i += 1) is still to apply
     double beta;
     uint64_t seed;
 };
@@ -81,9 +84,16 @@ __global__ void __launch_bounds__(NT) kz(const Ctl* ctl, const cx<double>* src, 
     }
 }
 
+// the previous iteration's end (:393-399 of the reference's loop), then
+// this iteration's start
 __global__ void k_begin(State* st)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (st->pend) {
+        st->pend = 0;
+        if (st->j >= 100 && st->j > 200) st->j = 0;
+        st->i += 1;
+    }
     st->xctl.done = 1;
     if (st->ctl.done) return;
     st->j += 1;
@@ -145,7 +155,7 @@ __global__ void __launch_bounds__(NT) k_err(State* st, const double* part, int n
     }
 }
 
-__global__ void k_project(const State* st, const cx<double>* MH, cx<double>* mT, long long N, double xyz)
+__global__ void k_project(State* st, const cx<double>* MH, cx<double>* mT, long long N, double xyz)
 {
     if (st->ctl.done) return;
     const int j = st->j;
@@ -165,13 +175,10 @@ __global__ void k_project(const State* st, const cx<double>* MH, cx<double>* mT,
         }
         mT[s] = h;
     }
-}
-
-__global__ void k_end(State* st)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0 || st->ctl.done) return;
-    if (st->j >= 100 && st->j > 200) st->j = 0;
-    st->i += 1;
+    // the iteration's end is applied by the next k_begin (no block here
+    // reads pend, and the counters it moves are read by no block here
+    // before they change)
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->pend = 1;
 }
 
 __global__ void __launch_bounds__(NT) k_count(const cx<double>* mT, double thr, double* part, long long N)
@@ -348,7 +355,6 @@ struct Plan {
         hipLaunchKernelGGL(k_err, dim3(1), dim3(NT), 0, st, state, part, nb, errd, 1.0 / (double)N);
         fft3(&state->ctl, MH, MH, 1);
         hipLaunchKernelGGL(k_project, dim3(nb), dim3(NT), 0, st, state, MH, mT, N, (double)N);
-        hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, st, state);
     }
 
     void set_initial(const double* m)
@@ -380,7 +386,7 @@ struct Plan {
         // chunks of iterations in captured graphs, replayed until done: one
         // with the re-randomisation path, one without, chosen per chunk from
         // the iteration counter (it advances by one per iteration until done)
-        const int ck = 8;
+        const int ck = 16;
         struct Guard {
             hipGraph_t g[2] = {nullptr, nullptr};
             hipGraphExec_t x[2] = {nullptr, nullptr};
@@ -400,9 +406,11 @@ struct Plan {
         }
         const long long max_replays = (long long)n / ck + 2;
         for (long long r = 0; r <= max_replays; ++r) {
-            // iteration i re-randomises when (i + 1) % 150 == 0 (k_begin)
+            // iteration i re-randomises when (i + 1) % 150 == 0 (k_begin);
+            // a pending end still moves the counter by one
+            const int i0 = h.i + (h.pend ? 1 : 0);
             bool rr = false;
-            for (int k = 0; k < ck; ++k) rr = rr || (h.i + k + 1) % 150 == 0;
+            for (int k = 0; k < ck; ++k) rr = rr || (i0 + k + 1) % 150 == 0;
             HCHK(hipGraphLaunch(gr.x[rr ? 1 : 0], st));
             HCHK(hipMemcpyAsync(&h, state, sizeof h, hipMemcpyDeviceToHost, st));
             HCHK(hipStreamSynchronize(st));
