@@ -561,6 +561,14 @@ void dgrow(P*& p, size_t& cap, size_t count)
     cap = count;
 }
 
+// a buffer of fixed size for the handle's lifetime
+template <typename P>
+void dalloc(P*& p, size_t count)
+{
+    size_t cap = 0;
+    dgrow(p, cap, count);
+}
+
 // stage boundaries timed on the handle's stream
 enum Stage { ST_INIT, ST_LINK, ST_ENUM, ST_SIZES, ST_SORT, ST_RELABEL, ST_N };
 
@@ -737,14 +745,13 @@ int m3d_perc_create(const int32_t* rve, int nx, int ny, int nz, int noOfPhases, 
         h->n = (long long)nx * ny * nz;
         HCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
         for (hipEvent_t& e : h->ev) HCHK(hipEventCreate(&e));
-        size_t c = 0;
-        perc::dgrow(h->ids, c, (size_t)h->n);
-        perc::dgrow(h->lab, c, (size_t)h->n);
-        perc::dgrow(h->qm, c, (size_t)h->n);
-        perc::dgrow(h->c1, c, (size_t)h->n);
-        perc::dgrow(h->flag, c, (size_t)h->n);
-        perc::dgrow(h->red, c, 4);
-        perc::dgrow(h->tot, c, 4);
+        perc::dalloc(h->ids, (size_t)h->n);
+        perc::dalloc(h->lab, (size_t)h->n);
+        perc::dalloc(h->qm, (size_t)h->n);
+        perc::dalloc(h->c1, (size_t)h->n);
+        perc::dalloc(h->flag, (size_t)h->n);
+        perc::dalloc(h->red, 4);
+        perc::dalloc(h->tot, 4);
         const size_t cb = (size_t)blocks(h->n, perc::BT);
         perc::dgrow(h->ci, h->cap_ci, cb);
         perc::dgrow(h->cr, h->cap_cr, cb);
