@@ -2,13 +2,16 @@
 // the basic scheme on complex 2D fields [E11, E22, E12] (tensor shear) with
 // a fixed 25 iterations per runAnalysis.
 //
-// Per iteration:
-//   polar : sigma = C(phase) eps (multipCx, :96-106); tau = sigma - C0 eps (:240)
-//   y, x  : forward 2D FFT of the three interleaved components (p3, :29-50)
-//   green : t = -G(xi) t with the reference's 6-entry Green operator per
-//           frequency (setGF :53-93, recomputed here; x > (Nx-1)/2 maps to
-//           -(Ny - x) as in the reference), DC = E xy (multiplyGF :114-141)
-//   x, y  : backward 2D FFT (p4); eps = t / xy (FFTBCKWRD_epsilon :143-148)
+// Per iteration (four launches):
+//   polar : eps = t / xy of the previous iteration (FFTBCKWRD_epsilon
+//           :143-148); sigma = C(phase) eps (multipCx, :96-106); tau =
+//           sigma - C0 eps (:240)
+//   y     : forward FFT along y of the three interleaved components (p3)
+//   x     : forward FFT along x, then t = -G(xi) t with the reference's
+//           6-entry Green operator per frequency (setGF :53-93, recomputed
+//           here; x > (Nx-1)/2 maps to -(Ny - x) as in the reference), DC =
+//           E xy (multiplyGF :114-141), then the backward FFT along x
+//   y     : backward FFT along y (p4)
 // convergenceCheck (:150-187) only prints, and its sum reads past the two-entry
 // frequency vector; its iteration counter is kept, its transform is not.
 // The 1D transforms are the y/x passes of m3d_fast.hpp on a three-component
@@ -25,6 +28,11 @@ namespace p2 {
 
 struct Comp3 {
     static constexpr int C = 3;
+    // the x pass with the Green step on the view of an Nx x Ny grid as
+    // g.Nx x 1 x g.Nzh (kx: x frequency, kz: y frequency); prm = {la0, mu0},
+    // ctl.E = the load as (re, im) pairs
+    template <typename T>
+    static __device__ void green(const Geo& g, const Ctl& ctl, const T* prm, int kx, int, int kz, cx<T>* v);
 };
 
 constexpr int NT = 256;
@@ -34,12 +42,20 @@ struct Mat {
     double la0, mu0;
 };
 
-__global__ void k_polar(const uint8_t* ph, Mat m, const cx<double>* eps, cx<double>* sig, cx<double>* t, long long xy)
+// from_t: the previous iteration's strain is still the unscaled inverse
+// transform in t; eps = t / xy is stored first (the scaling step of the
+// previous iteration, done here to save a launch)
+__global__ void k_polar(const uint8_t* ph, Mat m, cx<double>* eps, cx<double>* sig, cx<double>* t, long long xy,
+                        int from_t)
 {
 #pragma clang fp contract(off)
     const long long s0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, st = (long long)gridDim.x * blockDim.x;
     for (long long i = s0; i < xy; i += st) {
         const double* C = ph[i] ? m.Cm : m.Ci;
+        if (from_t) {
+            const double n = (double)xy;
+            for (int c = 0; c < 3; ++c) eps[3 * i + c] = cx<double>{t[3 * i + c].re / n, t[3 * i + c].im / n};
+        }
         const cx<double> e0 = eps[3 * i], e1 = eps[3 * i + 1], e2 = eps[3 * i + 2];
         for (int r = 0; r < 3; ++r) {
             const cx<double> sg{(C[3 * r] * e0.re + C[3 * r + 1] * e1.re) + C[3 * r + 2] * e2.re,
@@ -54,45 +70,48 @@ __global__ void k_polar(const uint8_t* ph, Mat m, const cx<double>* eps, cx<doub
 
 __device__ __forceinline__ double dl(int i, int j) { return i == j ? 1.0 : 0.0; }
 
-__global__ void k_green(cx<double>* t, int Nx, int Ny, double la0, double mu0, cx<double> E0, cx<double> E1,
-                        cx<double> E2)
+// the Green step at frequency (x, y) of the Nx x Ny grid on its three values
+// v (multiplyGF, :114-141): DC = E xy, else v = -G(xi) v with the shear
+// weighted twice; E as (re, im) pairs
+__device__ __forceinline__ void green2d(int x, int y, int Nx, int Ny, double la0, double mu0, const double* E,
+                                        cx<double>* v)
 {
 #pragma clang fp contract(off)
-    const long long xy = (long long)Nx * Ny;
+    if (x == 0 && y == 0) {
+        const double n = (double)((long long)Nx * Ny);
+        for (int c = 0; c < 3; ++c) v[c] = cx<double>{E[2 * c] * n, E[2 * c + 1] * n};
+        return;
+    }
     const double d1 = 1 / (4 * mu0);
     const double d2 = (la0 + mu0) / (mu0 * (la0 + 2 * mu0));
     const int q[6][4] = {{0, 0, 0, 0}, {0, 0, 1, 1}, {0, 0, 0, 1}, {1, 1, 1, 1}, {1, 1, 0, 1}, {0, 1, 0, 1}};
     const int w[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    const long long s0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, st = (long long)gridDim.x * blockDim.x;
-    for (long long i = s0; i < xy; i += st) {
-        cx<double>* v = t + 3 * i;
-        if (i == 0) {
-            const double n = (double)xy;
-            v[0] = cx<double>{E0.re * n, E0.im * n};
-            v[1] = cx<double>{E1.re * n, E1.im * n};
-            v[2] = cx<double>{E2.re * n, E2.im * n};
-            continue;
-        }
-        const int x = (int)(i / Ny), y = (int)(i % Ny);
-        const int x1 = (x <= (Nx - 1) / 2) ? x : -(Ny - x);
-        const int y1 = (y <= (Ny - 1) / 2) ? y : -(Ny - y);
-        const double zet[2] = {x1 * 1.0, y1 * 1.0};
-        const double n2 = zet[0] * zet[0] + zet[1] * zet[1];
-        double G[6];
-        for (int r = 0; r < 6; ++r) {
-            const int a = q[r][0], b = q[r][1], k = q[r][2], h = q[r][3];
-            G[r] = (d1 / n2) * (((dl(k, a) * zet[h] * zet[b] + dl(h, a) * zet[k] * zet[b]) + dl(k, b) * zet[h] * zet[a]) +
-                                dl(h, b) * zet[k] * zet[a]) -
-                   (d2 * zet[a] * zet[b] * zet[k] * zet[h]) / (n2 * n2);
-        }
-        const cx<double> a0 = v[0], a1 = v[1], a2 = v[2];
-        for (int j = 0; j < 3; ++j) {
-            const double re = -((G[w[j][0]] * a0.re + G[w[j][1]] * a1.re) + G[w[j][2]] * a2.re * 2);
-            const double im = -((G[w[j][0]] * a0.im + G[w[j][1]] * a1.im) + G[w[j][2]] * a2.im * 2);
-            v[j] = cx<double>{re, im};
-        }
+    const int x1 = (x <= (Nx - 1) / 2) ? x : -(Ny - x);
+    const int y1 = (y <= (Ny - 1) / 2) ? y : -(Ny - y);
+    const double zet[2] = {x1 * 1.0, y1 * 1.0};
+    const double n2 = zet[0] * zet[0] + zet[1] * zet[1];
+    double G[6];
+    for (int r = 0; r < 6; ++r) {
+        const int a = q[r][0], b = q[r][1], k = q[r][2], h = q[r][3];
+        G[r] = (d1 / n2) * (((dl(k, a) * zet[h] * zet[b] + dl(h, a) * zet[k] * zet[b]) + dl(k, b) * zet[h] * zet[a]) +
+                            dl(h, b) * zet[k] * zet[a]) -
+               (d2 * zet[a] * zet[b] * zet[k] * zet[h]) / (n2 * n2);
+    }
+    const cx<double> a0 = v[0], a1 = v[1], a2 = v[2];
+    for (int j = 0; j < 3; ++j) {
+        const double re = -((G[w[j][0]] * a0.re + G[w[j][1]] * a1.re) + G[w[j][2]] * a2.re * 2);
+        const double im = -((G[w[j][0]] * a0.im + G[w[j][1]] * a1.im) + G[w[j][2]] * a2.im * 2);
+        v[j] = cx<double>{re, im};
     }
 }
+
+template <typename T>
+__device__ void Comp3::green(const Geo& g, const Ctl& ctl, const T* prm, int kx, int, int kz, cx<T>* v)
+{
+    green2d(kx, kz, g.Nx, g.Nzh, prm[0], prm[1], ctl.E, v);
+}
+
+static_assert(fast::has_green_hook<Comp3>::value, "the 2D Green step must replace the 3D one in the x pass");
 
 __global__ void k_scale(const cx<double>* t, cx<double>* eps, long long n3, double xy)
 {
@@ -111,9 +130,11 @@ struct Solver {
     hipStream_t st = nullptr;
     cx<double>*eps = nullptr, *sig = nullptr, *t = nullptr;
     uint8_t* ph = nullptr;
-    Ctl* ctl = nullptr; // never done: gates nothing in the passes
+    Ctl* ctl = nullptr; // never done: gates nothing in the passes; E holds the load
     cx<double>*twx = nullptr, *twy = nullptr;
-    Geo gy{}, gx{};
+    double* dprm = nullptr; // {la0, mu0} for the Green step
+    int* drve = nullptr;    // {0}: the one solve's parameter row
+    Geo gy{}, gx1{};
     Mat m{};
     cx<double> E[3] = {{0, 0}, {0, 0}, {0, 0}};
     bool have_mat = false;
@@ -142,6 +163,9 @@ struct Solver {
         alloc(&t, sizeof(cx<double>) * 3 * xy);
         alloc(&ph, (size_t)xy);
         alloc(&ctl, sizeof(Ctl));
+        alloc(&dprm, sizeof(double) * 2);
+        alloc(&drve, sizeof(int));
+        HCHK(hipMemset(drve, 0, sizeof(int)));
         HCHK(hipMemset(eps, 0, sizeof(cx<double>) * 3 * xy));
         HCHK(hipMemset(sig, 0, sizeof(cx<double>) * 3 * xy));
         HCHK(hipMemset(ctl, 0, sizeof(Ctl)));
@@ -156,15 +180,18 @@ struct Solver {
         std::memset(&gy, 0, sizeof gy);
         gy.Nx = nx; gy.Ny = ny; gy.Nz = 1; gy.Nzh = 1; gy.C = 3; gy.nsolve = 1;
         gy.sy = 3; gy.sxa = (long long)ny * 3; gy.sxb = 0; gy.lxb = 0; gy.xbm = 0;
-        gx = gy;
-        gx.Nx = 1; gx.Ny = nx; gx.Nzh = ny;
-        gx.sy = (long long)ny * 3; gx.sxa = 0;
+        // x lines with the Green step: the grid as nx x 1 x ny, rows x at
+        // stride ny*3, the y points in the role of the kz tile
+        gx1 = gy;
+        gx1.Nx = nx; gx1.Ny = 1; gx1.Nzh = ny;
+        gx1.sy = (long long)ny * 3; gx1.sxa = (long long)ny * 3; gx1.sxb = 0; gx1.lxb = 0; gx1.xbm = 0;
     }
 
     ~Solver()
     {
         if (st) (void)hipStreamSynchronize(st);
-        for (void* p : {(void*)eps, (void*)sig, (void*)t, (void*)ph, (void*)ctl, (void*)twx, (void*)twy})
+        for (void* p : {(void*)eps, (void*)sig, (void*)t, (void*)ph, (void*)ctl, (void*)dprm, (void*)drve, (void*)twx,
+                        (void*)twy})
             if (p) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -194,14 +221,32 @@ struct Solver {
               pass_at<48, 1, DIR>(gy, twy) || pass_at<32, 1, DIR>(gy, twy)))
             throw ArgError{M3D_ERR_UNSUPPORTED, "FFT_Analysis2D: no y kernel"};
     }
-    template <int DIR>
-    void xpass()
+    // x lines: forward FFT, the Green step (Comp3::green), inverse FFT in one
+    // pass over each tile
+    template <int L>
+    bool xg_at()
     {
-#define M3D_2DX(L) pass_at<L, fast::xytile<double>(L, 3), DIR>(gx, twx)
-        if (!(M3D_2DX(512) || M3D_2DX(256) || M3D_2DX(192) || M3D_2DX(128) || M3D_2DX(96) || M3D_2DX(64) ||
-              M3D_2DX(48) || M3D_2DX(32)))
+        if (gx1.Nx != L) return false;
+        constexpr int TK = fast::xtile<double>(L, 3);
+        const int nkt = (gx1.Nzh + TK - 1) / TK;
+        const size_t sm = ((size_t)L * TK * 3 + L) * sizeof(cx<double>);
+        if (sm > 65536) throw ArgError{M3D_ERR_UNSUPPORTED, "FFT_Analysis2D: x tile too large"};
+        Ptr<double> p;
+        std::memset(&p, 0, sizeof p);
+        p.S = t;
+        p.K = t;
+        p.twx = twx;
+        p.ctl = ctl;
+        p.rve_of = drve;
+        hipLaunchKernelGGL((fast::kxy<double, Comp3, 1, -1, BUF_MAIN, L, TK>), dim3(nkt), dim3(NT), sm, st, gx1, p,
+                           (const double*)dprm);
+        return true;
+    }
+    void xgreen()
+    {
+        if (!(xg_at<512>() || xg_at<256>() || xg_at<192>() || xg_at<128>() || xg_at<96>() || xg_at<64>() ||
+              xg_at<48>() || xg_at<32>()))
             throw ArgError{M3D_ERR_UNSUPPORTED, "FFT_Analysis2D: no x kernel"};
-#undef M3D_2DX
     }
 
     // material(properties, RVE1), :189-221
@@ -220,6 +265,8 @@ struct Solver {
         }
         m.la0 = alf * (lai + lam) / 2.;
         m.mu0 = alf * (mui + mum) / 2.;
+        const double lm[2] = {m.la0, m.mu0};
+        HCHK(hipMemcpy(dprm, lm, sizeof lm, hipMemcpyHostToDevice));
         std::vector<uint8_t> p((size_t)xy);
         for (long long i = 0; i < xy; ++i) p[(size_t)i] = rve[i] == 1 ? 1 : 0; // RVE(i)==1: matrix (:99-103)
         HCHK(hipMemcpy(ph, p.data(), (size_t)xy, hipMemcpyHostToDevice));
@@ -230,16 +277,20 @@ struct Solver {
     void run()
     {
         if (!have_mat) throw ArgError{M3D_ERR_STATE, "runAnalysis: material() first"};
+        double e_ri[6];
+        for (int c = 0; c < 3; ++c) {
+            e_ri[2 * c] = E[c].re;
+            e_ri[2 * c + 1] = E[c].im;
+        }
+        HCHK(hipMemcpy(reinterpret_cast<char*>(ctl) + offsetof(Ctl, E), e_ri, sizeof e_ri, hipMemcpyHostToDevice));
         for (int it = 0; it < 25; ++it) {
-            hipLaunchKernelGGL(k_polar, dim3(nb), dim3(NT), 0, st, ph, m, eps, sig, t, xy);
+            hipLaunchKernelGGL(k_polar, dim3(nb), dim3(NT), 0, st, ph, m, eps, sig, t, xy, it > 0 ? 1 : 0);
             ypass<-1>();
-            xpass<-1>();
-            hipLaunchKernelGGL(k_green, dim3(nb), dim3(NT), 0, st, t, Nx, Ny, m.la0, m.mu0, E[0], E[1], E[2]);
-            xpass<1>();
+            xgreen();
             ypass<1>();
-            hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, st, t, eps, 3 * xy, (double)xy);
             itNo += 1;
         }
+        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, st, t, eps, 3 * xy, (double)xy);
         HCHK(hipGetLastError());
         HCHK(hipStreamSynchronize(st));
     }

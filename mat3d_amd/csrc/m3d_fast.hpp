@@ -684,6 +684,14 @@ __device__ __forceinline__ const cx<T>* xy_rows(const Geo& g, const Ptr<T>& p, i
     return (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + (long long)o * g.sy + (long long)kz0 * C;
 }
 
+// a physics type may bring its own Green step for the x pass:
+// PH::green<T>(g, ctl, prm, kx, y, kz, v) on one frequency's values in LDS,
+// DC included
+template <typename P, typename = void>
+struct has_green_hook : std::false_type {};
+template <typename P>
+struct has_green_hook<P, std::void_t<decltype(&P::template green<double>)>> : std::true_type {};
+
 // a tile already in LDS: forward FFT along the lines, then the Green step and
 // the inverse FFT (KIND 1) or the equilibrium partial sums (KIND 2; t is the
 // tile's index within its solve).  Returns with the tile in LDS (KIND 0, 1).
@@ -702,7 +710,9 @@ __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl
             const int kx = f / nk, kzl = f - kx * nk;
             const int kz = kz0 + kzl;
             cx<T>* v = sl + kx * W + kzl * C;
-            if (kx == 0 && y == 0 && kz == 0) {
+            if constexpr (has_green_hook<PH>::value) {
+                PH::template green<T>(g, ctl, prm, kx, y, kz, v);
+            } else if (kx == 0 && y == 0 && kz == 0) {
 #pragma unroll
                 for (int c = 0; c < C; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
             } else {
