@@ -12,9 +12,9 @@
 //   phase   : err partials sum |(|MT|) - normC| (funcNormC); ang = arg(MT), or
 //             0.3 arg(X) + 0.7 arg(MT); MH = normC e^{i ang}
 //   err     : trackrealerrorA(i) = err / xyz; err < 0.1 -> done = 1
-//   inv FFT : MH = IFFT3(MH) (unnormalised)
-//   project : mH = MH / xyz; j < 100: the reference's relaxed projection with
-//             beta, else clamp to [0, 1]; mT = mH
+//   inv FFT : IFFT3(MH) (unnormalised; x, y, then z), the z pass storing
+//             the projection: mH = MH / xyz; j < 100: the reference's relaxed
+//             projection with beta, else clamp to [0, 1]; mT = mH
 //   end     : j > 200 -> j = 0 (after a clamp iteration); itNo += 1 --
 //             marked by the projection, applied by the next begin
 // The 3D complex FFT is z-lines in LDS (contiguous, tiles of TY lines), then
@@ -61,8 +61,29 @@ struct State {
 };
 
 // z lines of length L, TY lines per block (line l at l*L), LDS [z][line]
-template <int L, int TY, int DIR>
-__global__ void __launch_bounds__(NT) kz(const Ctl* ctl, const cx<double>* src, cx<double>* dst, const cx<double>* twz)
+// mH = MH / xyz; j < 100: the reference's relaxed projection with beta
+// (t: the current mT), else clamp to [0, 1]
+__device__ __forceinline__ cx<double> project(cx<double> MH, double t, int j, double beta, double xyz)
+{
+    cx<double> h{MH.re / xyz, MH.im / xyz};
+    if (j < 100) {
+        if (h.re < 0) h = cx<double>{t - beta * h.re, 0.0};
+        if (h.re > 1) h = cx<double>{t - beta * (h.re - 1), 0.0};
+        if (h.re > 1) h = cx<double>{1.0, 0.0};
+    } else {
+        const double re = h.re;
+        if (re < 0) h = cx<double>{0.0, 0.0};
+        if (re > 1) h = cx<double>{1.0, 0.0};
+    }
+    return h;
+}
+
+// PROJ: the last pass of the inverse transform, stored through the
+// projection (project()) into dst = mT (each element read and rewritten by
+// one thread)
+template <int L, int TY, int DIR, bool PROJ>
+__global__ void __launch_bounds__(NT) kz(const Ctl* ctl, const cx<double>* src, cx<double>* dst, const cx<double>* twz,
+                                         State* stt, double xyz)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<double>* sl = reinterpret_cast<cx<double>*>(smem_raw);
@@ -78,9 +99,21 @@ __global__ void __launch_bounds__(NT) kz(const Ctl* ctl, const cx<double>* src, 
     __syncthreads();
     fast::fft<double, L, TY, NT, DIR, 1>(sl, tw);
     cx<double>* d = dst + l0 * L;
-    for (int e = threadIdx.x; e < TY * L; e += NT) {
-        const int line = e / L, z = e - line * L;
-        d[e] = sl[z * TY + line];
+    if constexpr (PROJ) {
+        const int j = stt->j;
+        const double beta = stt->beta;
+        for (int e = threadIdx.x; e < TY * L; e += NT) {
+            const int line = e / L, z = e - line * L;
+            d[e] = project(sl[z * TY + line], d[e].re, j, beta, xyz);
+        }
+        // the iteration's end is applied by the next k_begin (no block here
+        // reads pend; j was read above)
+        if (blockIdx.x == 0 && threadIdx.x == 0) stt->pend = 1;
+    } else {
+        for (int e = threadIdx.x; e < TY * L; e += NT) {
+            const int line = e / L, z = e - line * L;
+            d[e] = sl[z * TY + line];
+        }
     }
 }
 
@@ -155,31 +188,6 @@ __global__ void __launch_bounds__(NT) k_err(State* st, const double* part, int n
     }
 }
 
-__global__ void k_project(State* st, const cx<double>* MH, cx<double>* mT, long long N, double xyz)
-{
-    if (st->ctl.done) return;
-    const int j = st->j;
-    const double beta = st->beta;
-    const long long s0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, step = (long long)gridDim.x * blockDim.x;
-    for (long long s = s0; s < N; s += step) {
-        cx<double> h{MH[s].re / xyz, MH[s].im / xyz};
-        if (j < 100) {
-            const double t = mT[s].re;
-            if (h.re < 0) h = cx<double>{t - beta * h.re, 0.0};
-            if (h.re > 1) h = cx<double>{t - beta * (h.re - 1), 0.0};
-            if (h.re > 1) h = cx<double>{1.0, 0.0};
-        } else {
-            const double re = h.re;
-            if (re < 0) h = cx<double>{0.0, 0.0};
-            if (re > 1) h = cx<double>{1.0, 0.0};
-        }
-        mT[s] = h;
-    }
-    // the iteration's end is applied by the next k_begin (no block here
-    // reads pend, and the counters it moves are read by no block here
-    // before they change)
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->pend = 1;
-}
 
 __global__ void __launch_bounds__(NT) k_count(const cx<double>* mT, double thr, double* part, long long N)
 {
@@ -280,22 +288,25 @@ struct Plan {
     }
 
     template <int L>
-    bool z_at(const Ctl* c, const cx<double>* src, cx<double>* dst, int dir)
+    bool z_at(const Ctl* c, const cx<double>* src, cx<double>* dst, int dir, bool proj)
     {
         if (Nz != L) return false;
         constexpr int TY = zlines(L);
         const long long lines = (long long)Nx * Ny;
         if (lines % TY) throw ArgError{M3D_ERR_UNSUPPORTED, "recPlan: Nx*Ny must be a multiple of the z tile"};
         const size_t sm = ((size_t)L * TY + L) * sizeof(cx<double>);
-        if (dir < 0) hipLaunchKernelGGL((kz<L, TY, -1>), dim3(lines / TY), dim3(NT), sm, st, c, src, dst, tw[0]);
-        else hipLaunchKernelGGL((kz<L, TY, 1>), dim3(lines / TY), dim3(NT), sm, st, c, src, dst, tw[0]);
+        const dim3 grid(lines / TY);
+        const double n = (double)N;
+        if (dir < 0) hipLaunchKernelGGL((kz<L, TY, -1, false>), grid, dim3(NT), sm, st, c, src, dst, tw[0], state, n);
+        else if (!proj) hipLaunchKernelGGL((kz<L, TY, 1, false>), grid, dim3(NT), sm, st, c, src, dst, tw[0], state, n);
+        else hipLaunchKernelGGL((kz<L, TY, 1, true>), grid, dim3(NT), sm, st, c, src, dst, tw[0], state, n);
         return true;
     }
-    void zpass(const Ctl* c, const cx<double>* src, cx<double>* dst, int dir)
+    void zpass(const Ctl* c, const cx<double>* src, cx<double>* dst, int dir, bool proj = false)
     {
-        if (!(z_at<256>(c, src, dst, dir) || z_at<192>(c, src, dst, dir) || z_at<128>(c, src, dst, dir) ||
-              z_at<96>(c, src, dst, dir) || z_at<64>(c, src, dst, dir) || z_at<48>(c, src, dst, dir) ||
-              z_at<32>(c, src, dst, dir)))
+        if (!(z_at<256>(c, src, dst, dir, proj) || z_at<192>(c, src, dst, dir, proj) ||
+              z_at<128>(c, src, dst, dir, proj) || z_at<96>(c, src, dst, dir, proj) ||
+              z_at<64>(c, src, dst, dir, proj) || z_at<48>(c, src, dst, dir, proj) || z_at<32>(c, src, dst, dir, proj)))
             throw ArgError{M3D_ERR_UNSUPPORTED, "recPlan: no z kernel"};
     }
 
@@ -353,8 +364,10 @@ struct Plan {
         }
         hipLaunchKernelGGL(k_phase, dim3(nb), dim3(NT), 0, st, state, MT, X, normC, MH, part, N);
         hipLaunchKernelGGL(k_err, dim3(1), dim3(NT), 0, st, state, part, nb, errd, 1.0 / (double)N);
-        fft3(&state->ctl, MH, MH, 1);
-        hipLaunchKernelGGL(k_project, dim3(nb), dim3(NT), 0, st, state, MH, mT, N, (double)N);
+        // inverse transform as x, y, then z with the projection into mT
+        xypass<1>(gx, &state->ctl, MH, tw[2]);
+        xypass<1>(gy, &state->ctl, MH, tw[1]);
+        zpass(&state->ctl, MH, mT, 1, true);
     }
 
     void set_initial(const double* m)
