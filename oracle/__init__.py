@@ -99,6 +99,7 @@ def lib():
             "m3o_perc_create": (vp, [i32p, ip, ip, ip, ip, ip]),
             "m3o_perc_destroy": (None, [vp]),
             "m3o_perc_run": (None, [vp]),
+            "m3o_perc_run_rounds": (None, [vp, ip]),
             "m3o_perc_result": (ip, [vp, i32p, ipp, dp, dp, ip]),
             "m3o_perc_surface": (ip, [vp, i32p, ip, ip]),
             "m3o_perc_fixed_point": (ip, [vp]),
@@ -437,11 +438,12 @@ class Percolation:
     def __del__(self):
         self.close()
 
-    def run(self):
-        """-> (C1, all_clusters, clusters_vol_frac, vol_intrstd_phs, fixed point reached)"""
+    def run(self, rounds=5):
+        """-> (C1, all_clusters, clusters_vol_frac, vol_intrstd_phs, fixed point
+        reached); rounds: rounds of the 8 x 6 sweeps (the reference runs 5)"""
         if len(set(self.shape)) != 1:
             raise ValueError("the reference's sweeps need Nx = Ny = Nz")
-        lib().m3o_perc_run(self._h)
+        lib().m3o_perc_run_rounds(self._h, int(rounds))
         n = int(np.prod(self.shape))
         c1 = np.empty(n, dtype=np.int32)
         cl = np.zeros(n + 1, dtype=np.int32)

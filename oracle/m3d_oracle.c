@@ -1916,11 +1916,17 @@ static int perc_cmp_desc(const void *a, const void *b)
  * permutations of the trigger test); then the clusters in first-appearance
  * order, their volume fractions, and every interest voxel relabelled
  * phasePlus + rank of its cluster's fraction (equal fractions, equal rank) */
-void m3o_perc_run(m3o_perc *p)
+void m3o_perc_run_rounds(m3o_perc *p, int rounds);
+
+void m3o_perc_run(m3o_perc *p) { m3o_perc_run_rounds(p, 5); }
+
+/* the same with `rounds` rounds of the 8 x 6 sweeps instead of the
+ * reference's fixed 5 (test use: how many rounds reach the fixed point) */
+void m3o_perc_run_rounds(m3o_perc *p, int rounds)
 {
     perc_setMargin(p, 0);
     const int lo = 1, hiX = p->Nx - 2, hiY = p->Ny - 2, hiZ = p->Nz - 2;
-    for (int X = 0; X < 5; ++X)
+    for (int X = 0; X < rounds; ++X)
         for (int s = 0; s < 8; ++s) {
             const int di = (s & 4) ? -1 : 1, dj = (s & 2) ? -1 : 1, dk = (s & 1) ? -1 : 1;
             for (int t = 1; t < 7; ++t)

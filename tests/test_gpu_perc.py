@@ -94,6 +94,13 @@ def test_winding_path_is_one_cluster():
     e = expected(ids, 2, 2)
     assert p.all_clusters == e[1].tolist() == [3]
     assert np.array_equal(p.C1, e[0])
+    # the reference's own sweeps, with rounds enough to reach their fixed
+    # point, give the GPU's result bit for bit
+    rounds = 10
+    while not (o := oracle.Percolation(ids, 2, 2).run(rounds))[4]:
+        rounds *= 2
+        assert rounds <= 320
+    same(p, o)
 
 
 def test_run_twice_and_rerun_after_surface():

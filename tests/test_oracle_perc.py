@@ -126,3 +126,20 @@ def test_surface_random_against_numpy():
     p = oracle.Percolation(ids, 2, 1)
     for a, b in ((0, 1), (1, 2), (2, 0), (1, 1)):
         assert p.surface(ids, a, b) == surface(ids, a, b)
+
+
+def test_more_rounds_reach_the_rule_graph_on_the_winding_path():
+    # the reference's own sweeps, given rounds enough, end at the connected
+    # sets of the rule graph (one cluster) on the path where 5 rounds stop short
+    f = snake(16)
+    ids = np.where(f, 2, 1).astype(np.int32)
+    e = expected(ids, 2, 2)
+    rounds = 5
+    while True:
+        c1, cl, fr, vol, fp = oracle.Percolation(ids, 2, 2).run(rounds)
+        if fp:
+            break
+        rounds *= 2
+        assert rounds <= 320
+    assert rounds > 5
+    assert np.array_equal(c1, e[0]) and cl.tolist() == e[1].tolist() == [3]
