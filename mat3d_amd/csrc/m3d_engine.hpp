@@ -40,6 +40,11 @@ constexpr size_t LDS_MAX = 65536;
 #define M3D_XPAD 128
 #endif
 
+// workgroups of a check-pass launch (the check kernels loop over their tiles)
+#ifndef M3D_CHK_WG
+#define M3D_CHK_WG 4096
+#endif
+
 struct HipError {
     hipError_t e;
     const char* what;
@@ -515,7 +520,9 @@ struct Engine {
         constexpr int TYc = fast::zlines<T>(LZ, C, 2 * LZ);
         if (g.TY != TYc) throw ArgError{M3D_ERR_UNSUPPORTED, "z tile mismatch"};
         const size_t sm = ((size_t)(LZ + 1) * TYc * C + 2 * LZ) * sizeof(cx<T>);
-        hipLaunchKernelGGL((fast::kz<T, PH, MODE, LZ, TYc>), dim3(g.nsolve * g.nbz), dim3(fast::ZNT), sm, st, g, p);
+        hipLaunchKernelGGL((fast::kz<T, PH, MODE, LZ, TYc>), dim3(grid_of(g.nsolve * g.nbz, MODE == MODE_CHK_CUR ||
+                                                                                            MODE == MODE_CHK_PREV)),
+                           dim3(fast::ZNT), sm, st, g, p);
         return true;
     }
 
@@ -540,19 +547,28 @@ struct Engine {
         else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
 
+    // workgroups of a launch: one per tile, or for the check passes (which
+    // loop over their tiles) at most M3D_CHK_WG
+    static unsigned grid_of(long long tiles, bool check)
+    {
+        return (unsigned)(check ? std::min<long long>(tiles, M3D_CHK_WG) : tiles);
+    }
+
     template <typename T, int KIND, int DIR, int WHICH, int L, int TK, int NTH>
     void xy_launch(const Geo& g, const Ptr<T>& p)
     {
         const int nkt = (g.Nzh + TK - 1) / TK;
         const int outer = KIND == 0 ? g.Nx : g.Ny;
         const size_t sm = ((size_t)L * TK * C + L) * sizeof(cx<T>);
-        const dim3 grid(g.nsolve * outer * nkt);
+        const dim3 grid(grid_of(g.nsolve * outer * nkt, WHICH != BUF_MAIN));
+        constexpr bool LOOP = WHICH != BUF_MAIN;
         if (sm > 65536) {
             // gfx950: up to 160 KB of LDS per workgroup
-            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH>),
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
         }
-        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH>), grid, dim3(NTH), sm, st, g, p, prm<T>());
+        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP>), grid, dim3(NTH), sm, st, g, p,
+                           prm<T>());
     }
 
     // KIND 0: y pass, 1: x pass + Green, 2: x pass of the check FFT.  The FP32

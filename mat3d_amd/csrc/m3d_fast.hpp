@@ -185,6 +185,18 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return b;
 }
 
+// The same per solve: nb = nsolve * per tiles, solve-major; XCD k takes the
+// k-th contiguous eighth of every solve's tiles, so a batch in which only
+// some solves still run keeps all 8 XCDs busy.  Falls back to xcd_remap
+// when per is not a multiple of 8.
+__device__ __forceinline__ int xcd_split(int b, int nb, int per)
+{
+    if (per < 8 || (per & 7) || nb % per) return xcd_remap(b, nb);
+    const int ps = per >> 3;
+    const int j = b >> 3, s = j / ps;
+    return s * per + (b & 7) * ps + (j - s * ps);
+}
+
 // thread index as an opaque value: address arithmetic derived from it is
 // redone where it is used instead of being hoisted out of a tile loop and
 // held in registers across it
@@ -467,20 +479,16 @@ __device__ __forceinline__ void lds_to_zchunk(const cx<T>* __restrict__ sl, cx<T
 // Block: solve s, plane x, lines y0..y0+TY-1.  LDS: (LZ+1)*W complex lines
 // (W = TY*C, line l = yl*C + c) + the Nz twiddle table.
 template <typename T, typename PH, int MODE, int LZ, int TY>
-__global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
+__device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx<T>* sl, double* red)
 {
     constexpr int C = PH::C;
     constexpr int W = TY * C;
     constexpr int NZ = 2 * LZ;
     constexpr int NZH = LZ + 1;
     constexpr int NT = ZNT;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[NT / 64];
-    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw = sl + NZH * W;
 
     const int per = g.Nx * (g.Ny / TY);
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = b / per;
     const int t = b - s * per;
     const int x = t / (g.Ny / TY);
@@ -666,6 +674,44 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
     }
 }
 
+// one solve's check is needed (the check passes are captured in every
+// iteration and gated on these flags)
+template <int WHICH>
+__device__ __forceinline__ bool chk_needed(const Ctl& c)
+{
+    return !c.done && (WHICH == BUF_CHK_CUR ? c.need_cur : c.need_prev);
+}
+
+// The main passes run one workgroup per tile.  The check passes run a capped
+// grid that loops over the tiles: when no solve needs the check, the whole
+// launch ends after a few thousand workgroups instead of dispatching one per
+// tile only to exit.
+template <typename T, typename PH, int MODE, int LZ, int TY>
+__global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
+{
+    constexpr int NT = ZNT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NT / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    constexpr bool CHK = MODE == MODE_CHK_CUR || MODE == MODE_CHK_PREV;
+    if constexpr (CHK) {
+        constexpr int WHICH = MODE == MODE_CHK_CUR ? BUF_CHK_CUR : BUF_CHK_PREV;
+        bool any = false;
+        for (int s = 0; s < g.nsolve; ++s) any = any || chk_needed<WHICH>(p.ctl[s]);
+        if (!any) return;
+        const int total = g.nsolve * g.Nx * (g.Ny / TY);
+        // a workgroup stays on XCD blockIdx.x % 8 and the grid is a multiple
+        // of 8 (or covers every tile once): the remap hands each XCD its
+        // contiguous eighth of the tiles, as for the one-tile launches
+        for (int b = blockIdx.x; b < total; b += gridDim.x) {
+            kz_tile<T, PH, MODE, LZ, TY>(g, p, xcd_split(b, total, g.Nx * (g.Ny / TY)), sl, red);
+            __syncthreads();
+        }
+    } else {
+        kz_tile<T, PH, MODE, LZ, TY>(g, p, xcd_split(blockIdx.x, gridDim.x, g.Nx * (g.Ny / TY)), sl, red);
+    }
+}
+
 // --------------------------------------------------------- y / x passes
 // KIND 0: y pass (DIR), KIND 1: x pass with Green step, KIND 2: x pass of the check FFT
 
@@ -767,20 +813,17 @@ __device__ __forceinline__ bool xy_active(const Ctl& c)
 
 // one workgroup per tile
 template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>()>
-__global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+__device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* __restrict__ prm_all, int b, cx<T>* sl,
+                                        double* red)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
     constexpr int NT = NTH;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[NT / 64];
-    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw = sl + L * W;
 
     const int nkt = (g.Nzh + TK - 1) / TK;
     const int lines_out = KIND == 0 ? g.Nx : g.Ny; // outer index of the tile
     const int per = lines_out * nkt;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = b / per;
     const int t = b - s * per;
     const int o = t / nkt;
@@ -801,6 +844,33 @@ __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T
     if constexpr (KIND == 1) prm = prm_all + 2 * p.rve_of[s];
     xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT>(g, p, p.ctl[s], prm, sl, tw, red, s, o, kz0, nk, t);
     if constexpr (KIND != 2) lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
+}
+
+// one workgroup per tile; LOOP (the solver's check passes only): a capped
+// grid loops over the tiles of every solve and leaves at once when no solve
+// needs the check (see kz)
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>(),
+          bool LOOP = false>
+__global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[NTH / 64];
+    cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
+    static_assert(!LOOP || WHICH != BUF_MAIN, "only the check passes loop over their tiles");
+    if constexpr (LOOP) {
+        bool any = false;
+        for (int s = 0; s < g.nsolve; ++s) any = any || chk_needed<WHICH>(p.ctl[s]);
+        if (!any) return;
+        const int total = g.nsolve * (KIND == 0 ? g.Nx : g.Ny) * ((g.Nzh + TK - 1) / TK);
+        for (int b = blockIdx.x; b < total; b += gridDim.x) {
+            xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH>(g, p, prm_all,
+                                                          xcd_split(b, total, total / max(1, g.nsolve)), sl, red);
+            __syncthreads();
+        }
+    } else {
+        xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH>(g, p, prm_all,
+                                                          xcd_split(blockIdx.x, gridDim.x, (int)gridDim.x / max(1, g.nsolve)), sl, red);
+    }
 }
 
 } // namespace fast
