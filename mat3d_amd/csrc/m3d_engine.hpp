@@ -42,7 +42,7 @@ constexpr size_t LDS_MAX = 65536;
 
 // workgroups of a check-pass launch (the check kernels loop over their tiles)
 #ifndef M3D_CHK_WG
-#define M3D_CHK_WG 4096
+#define M3D_CHK_WG 32768
 #endif
 
 struct HipError {
