@@ -696,9 +696,9 @@ __global__ void __launch_bounds__(ZNT) M3D_KZ_ATTR kz(Geo g, Ptr<T> p)
     constexpr bool CHK = MODE == MODE_CHK_CUR || MODE == MODE_CHK_PREV;
     if constexpr (CHK) {
         constexpr int WHICH = MODE == MODE_CHK_CUR ? BUF_CHK_CUR : BUF_CHK_PREV;
-        bool any = false;
-        for (int s = 0; s < g.nsolve; ++s) any = any || chk_needed<WHICH>(p.ctl[s]);
-        if (!any) return;
+        int mine = 0;
+        for (int s = threadIdx.x; s < g.nsolve; s += NT) mine |= chk_needed<WHICH>(p.ctl[s]);
+        if (!__syncthreads_or(mine)) return;
         const int total = g.nsolve * g.Nx * (g.Ny / TY);
         // a workgroup stays on XCD blockIdx.x % 8 and the grid is a multiple
         // of 8 (or covers every tile once): the remap hands each XCD its
@@ -858,9 +858,9 @@ __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T
     cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
     static_assert(!LOOP || WHICH != BUF_MAIN, "only the check passes loop over their tiles");
     if constexpr (LOOP) {
-        bool any = false;
-        for (int s = 0; s < g.nsolve; ++s) any = any || chk_needed<WHICH>(p.ctl[s]);
-        if (!any) return;
+        int mine = 0;
+        for (int s = threadIdx.x; s < g.nsolve; s += NTH) mine |= chk_needed<WHICH>(p.ctl[s]);
+        if (!__syncthreads_or(mine)) return;
         const int total = g.nsolve * (KIND == 0 ? g.Nx : g.Ny) * ((g.Nzh + TK - 1) / TK);
         for (int b = blockIdx.x; b < total; b += gridDim.x) {
             xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH>(g, p, prm_all,
