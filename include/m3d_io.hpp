@@ -202,8 +202,50 @@ struct ElasticReport {
     const double *CHomo, *SHomo, *epHomo, *sigHomo;
 };
 
-// homogenization report, FFT_analysis3D.cpp:905-996 (Hashin-Shtrikman
-// bounds of the n-phase composite, Hashin & Shtrikman, JMPS 1963, as there)
+// ---------------------------------------------------------------- estimates
+// Isotropic effective moduli: bulk K, shear G and Young's E = 9KG/(3K+G).
+struct IsoModuli {
+    double K, G, E;
+};
+
+// Hashin-Shtrikman bound of an isotropic n-phase composite (Hashin &
+// Shtrikman, JMPS 11 (1963) 127-140) written as the comparison-medium average
+// of Walpole (JMPS 14 (1966) 151-162): for comparison moduli (Kc, Gc),
+//   K = [ sum_i c_i / (K_i + 4Gc/3) ]^-1 - 4Gc/3,
+//   G = [ sum_i c_i / (G_i + Zc) ]^-1 - Zc,   Zc = Gc (9Kc + 8Gc) / (6 (Kc + 2Gc)).
+// (Kc, Gc) = the smallest phase moduli gives the lower bound, the largest the
+// upper bound.  For fractions c_i summing to one this is algebraically the
+// 1963 paper's form (the homogenization report of FFT_analysis3D.cpp:905-942
+// prints that bound).
+inline IsoModuli hashin_shtrikman(const std::vector<double>& c, const std::vector<double>& K,
+                                  const std::vector<double>& G, double Kc, double Gc)
+{
+    const double kshift = 4.0 * Gc / 3.0;
+    const double gshift = Gc * (9.0 * Kc + 8.0 * Gc) / (6.0 * (Kc + 2.0 * Gc));
+    double kinv = 0.0, ginv = 0.0;
+    for (size_t i = 0; i < c.size(); ++i) {
+        kinv += c[i] / (K[i] + kshift);
+        ginv += c[i] / (G[i] + gshift);
+    }
+    IsoModuli m;
+    m.K = 1.0 / kinv - kshift;
+    m.G = 1.0 / ginv - gshift;
+    m.E = 9.0 * m.K * m.G / (3.0 * m.K + m.G);
+    return m;
+}
+
+// Conductivity of a matrix (km, fraction fm) with spherical inclusions (ki,
+// fraction fi) by the Hashin-Shtrikman comparison medium kc (Hashin &
+// Shtrikman, J. Appl. Phys. 33 (1962) 3125-3131):
+//   k(kc) = [ fm / (km + 2kc) + fi / (ki + 2kc) ]^-1 - 2kc.
+// kc = km is also the Mori-Tanaka (Maxwell Garnett) estimate of spheres in
+// the matrix; kc = min / max of (km, ki) give the lower / upper bounds.
+inline double hs_conductivity(double fm, double km, double fi, double ki, double kc)
+{
+    return 1.0 / (fm / (km + 2.0 * kc) + fi / (ki + 2.0 * kc)) - 2.0 * kc;
+}
+
+// homogenization report, FFT_analysis3D.cpp:905-996
 inline void write_report(std::ostream& homo, const ElasticReport& ph)
 {
     const ElasticReport& a = ph;
@@ -214,21 +256,10 @@ inline void write_report(std::ostream& homo, const ElasticReport& ph)
         la[i] = ph.v[i] * ph.El[i] / (1 + ph.v[i]) / (1 - 2 * ph.v[i]);
         K[i] = la[i] + (2. / 3.) * mu[i];
     }
-    const double Gn = *std::max_element(mu.begin(), mu.end()), G1 = *std::min_element(mu.begin(), mu.end());
-    const double Kn = *std::max_element(K.begin(), K.end()), K1 = *std::min_element(K.begin(), K.end());
-    const double alfa1 = -3 / (3 * K1 + 4 * G1), alfan = -3 / (3 * Kn + 4 * Gn);
-    const double beta1 = -(3 * (K1 + 2 * G1)) / ((3 * K1 + 4 * G1) * 5 * G1);
-    const double betan = -(3 * (Kn + 2 * Gn)) / ((3 * Kn + 4 * Gn) * 5 * Gn);
-    double A1 = 0, An = 0, B1 = 0, Bn = 0;
-    for (int i = 0; i < n; ++i) {
-        if (K[i] != K1) A1 += ph.vol[i] / ((1. / (K[i] - K1)) - alfa1);
-        if (K[i] != Kn) An += ph.vol[i] / ((1. / (K[i] - Kn)) - alfan);
-        if (mu[i] != G1) B1 += ph.vol[i] / ((0.5 / (mu[i] - G1)) - beta1);
-        if (mu[i] != Gn) Bn += ph.vol[i] / ((0.5 / (mu[i] - Gn)) - betan);
-    }
-    const double K_min = K1 + A1 / (1 + alfa1 * A1), K_max = Kn + An / (1 + alfan * An);
-    const double G_min = G1 + 0.5 * B1 / (1 + beta1 * B1), G_max = Gn + 0.5 * Bn / (1 + betan * Bn);
-    const double E_min = 9 * K_min * G_min / (3 * K_min + G_min), E_max = 9 * K_max * G_max / (3 * K_max + G_max);
+    const auto Kr = std::minmax_element(K.begin(), K.end());
+    const auto Gr = std::minmax_element(mu.begin(), mu.end());
+    const IsoModuli lo = hashin_shtrikman(ph.vol, K, mu, *Kr.first, *Gr.first);
+    const IsoModuli hi = hashin_shtrikman(ph.vol, K, mu, *Kr.second, *Gr.second);
     const char* rule = "      ----------------------------------------------------------------";
 
     homo << "*************************************************************************" << std::endl << std::endl;
@@ -246,8 +277,8 @@ inline void write_report(std::ostream& homo, const ElasticReport& ph)
     }
     homo << rule << std::endl << std::endl;
     homo << "                        Hashin-Shtrikman Bounds" << std::endl;
-    homo << "         E_min=" << E_min << "   " << "K_min=" << K_min << "   " << "G_min=" << G_min << std::endl;
-    homo << "         E_max=" << E_max << "   " << "K_max=" << K_max << "   " << "G_max=" << G_max << std::endl
+    homo << "         E_min=" << lo.E << "   " << "K_min=" << lo.K << "   " << "G_min=" << lo.G << std::endl;
+    homo << "         E_max=" << hi.E << "   " << "K_max=" << hi.K << "   " << "G_max=" << hi.G << std::endl
          << std::endl << std::endl;
     homo << "************************************************************************" << std::endl << std::endl;
     homo << "Homogenized Elastic Modules:" << std::endl;
@@ -301,15 +332,16 @@ inline void write_report(std::ostream& homo, const ThermalReport& a)
     const double km = a.km, ki = a.ki;
     const long long rve_sum = a.rve_sum;
     const long long xyz = (long long)a.Nx * a.Ny * a.Nz;
-    const double fi_m = (double)(rve_sum / xyz);
-    const double fi_i = 1 - fi_m;
-    const double k_mt = km + (fi_i * (ki - km)) / (1 + fi_m * (ki - km) / (3 * km));
-    const double k_up = fi_i * ki + fi_m * km - fi_i * fi_m * (km - ki) * (km - ki) / (3 * ki - fi_i * (ki - km));
-    const double k_low = fi_i * ki + fi_m * km - fi_i * fi_m * (km - ki) * (km - ki) / (3 * km + fi_m * (ki - km));
+    const double fmat = (double)(rve_sum / xyz);
+    // the report's three estimates (FFT_thermal3D.cpp:582-592): Mori-Tanaka
+    // and the "lower" value are the matrix-referenced form, the "upper" the
+    // inclusion-referenced one
+    const double k_matrix = hs_conductivity(fmat, km, 1.0 - fmat, ki, km);
+    const double k_incl = hs_conductivity(fmat, km, 1.0 - fmat, ki, ki);
     const double Km[9] = {km, 0, 0, 0, km, 0, 0, 0, km};
     const double Ki[9] = {ki, 0, 0, 0, ki, 0, 0, 0, ki};
     const double Rm[9] = {1 / km, 0, 0, 0, 1 / km, 0, 0, 0, 1 / km};
-    homo << "Volume Fraction of the White Phase (or 1)=" << fi_m << std::endl << std::endl;
+    homo << "Volume Fraction of the White Phase (or 1)=" << fmat << std::endl << std::endl;
     homo << "K phase matrix  ( or 1)=" << std::endl;
     print_matrix(homo, Km, 3, 3);
     homo << "     " << std::endl;
@@ -318,11 +350,11 @@ inline void write_report(std::ostream& homo, const ThermalReport& a)
     homo << "     " << std::endl;
     homo << "Nx=" << a.Nx << "   " << "Ny=" << a.Ny << "   " << a.Nz << std::endl << std::endl;
     homo << std::endl << "KHomogenized Mori-Tanaka=" << std::endl << std::endl;
-    homo << k_mt << std::endl;
+    homo << k_matrix << std::endl;
     homo << std::endl << "Lower bound HS=" << std::endl << std::endl;
-    homo << k_low << std::endl;
+    homo << k_matrix << std::endl;
     homo << std::endl << "Upper bound LS=" << std::endl << std::endl;
-    homo << k_up << std::endl;
+    homo << k_incl << std::endl;
     homo << std::endl << "KHomogenized=" << std::endl << std::endl;
     print_matrix(homo, a.KHomo, 3, 3);
     homo << std::endl;

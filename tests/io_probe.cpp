@@ -6,7 +6,9 @@
 //   io_probe report_e < spec        elastic report (spec: Nx Ny Nz strainBase nph,
 //                                   El[nph] v[nph] vol[nph], C S epHomo sigHomo)
 //   io_probe report_t < spec        thermal report (spec: Nx Ny Nz km ki rve_sum, K R G Q)
+//   io_probe bounds < spec          Hashin-Shtrikman bounds and conductivity estimates
 //   io_probe vti <file>             a 2x3x4 image, components (i, -i, 0.5 i)
+#include <algorithm>
 #include <cstring>
 #include <iostream>
 #include <vector>
@@ -54,6 +56,24 @@ int main(int argc, char** argv)
         for (auto& x : m) std::cin >> x;
         r.KHomo = m.data(); r.RHomo = m.data() + 9; r.tempGradHomo = m.data() + 18; r.heatFluxHomo = m.data() + 27;
         write_report(std::cout, r);
+    } else if (mode == "bounds") {
+        // spec: n c[n] K[n] G[n] fm km fi ki -> HS lower K G E, upper K G E,
+        // conductivity with the matrix, then the inclusion, as comparison medium
+        int n;
+        std::cin >> n;
+        std::vector<double> c(n), K(n), G(n);
+        for (auto& x : c) std::cin >> x;
+        for (auto& x : K) std::cin >> x;
+        for (auto& x : G) std::cin >> x;
+        double fm, km, fi, ki;
+        std::cin >> fm >> km >> fi >> ki;
+        const auto Kr = std::minmax_element(K.begin(), K.end());
+        const auto Gr = std::minmax_element(G.begin(), G.end());
+        const IsoModuli lo = hashin_shtrikman(c, K, G, *Kr.first, *Gr.first);
+        const IsoModuli hi = hashin_shtrikman(c, K, G, *Kr.second, *Gr.second);
+        std::cout.precision(17);
+        std::cout << lo.K << " " << lo.G << " " << lo.E << " " << hi.K << " " << hi.G << " " << hi.E << " "
+                  << hs_conductivity(fm, km, fi, ki, km) << " " << hs_conductivity(fm, km, fi, ki, ki) << "\n";
     } else if (mode == "vti") {
         std::vector<double> a(24 * 2), b(24);
         for (int i = 0; i < 24; ++i) { a[2 * i] = i; a[2 * i + 1] = -i; b[i] = 0.5 * i; }

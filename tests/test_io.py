@@ -114,6 +114,80 @@ def test_thermal_report_layout_and_estimates(probe):
     assert np.allclose(K, r["K"], rtol=1e-5)
 
 
+def _bounds(probe, c, K, G, fm, km, fi, ki):
+    spec = f"{len(c)} " + " ".join(repr(float(x)) for x in [*c, *K, *G, fm, km, fi, ki])
+    return [float(x) for x in run(probe, "bounds", stdin=spec).split()]
+
+
+def test_hs_bounds_against_the_oracle_many_phases(probe):
+    """The report's Hashin-Shtrikman bounds (comparison-medium form) equal the
+    oracle's restatement of FFT_analysis3D.cpp:911-942 on a 6-grain RVE."""
+    n = 8
+    ids = gen.voronoi(n, 6, seed=11)
+    rng = np.random.default_rng(4)
+    mat = np.zeros((6, 7))
+    mat[:, 0] = rng.uniform(50e9, 300e9, 6)
+    mat[:, 1] = rng.uniform(0.2, 0.35, 6)
+    mat[:, 2] = np.inf
+    mat[:, 4] = 1
+    o = oracle.Elastic(n, n, n, "e", nthreads=2)
+    o.material(mat, ids)
+    r = o.homogenization(1e-4, 1e-4)
+    vol = o.matrices(6)["vol"]
+    E, v = mat[:, 0], mat[:, 1]
+    G = E / (2 * (1 + v))
+    K = E / (3 * (1 - 2 * v))
+    b = _bounds(probe, vol, K, G, 0.5, 1.0, 0.5, 2.0)
+    hs = r["hs"]  # E_min K_min G_min E_max K_max G_max
+    got = [b[2], b[0], b[1], b[5], b[3], b[4]]
+    assert np.allclose(got, hs, rtol=1e-12, atol=0)
+    # the bounds bracket the computed effective bulk modulus (isotropic part)
+    C = r["C"]
+    Keff = C[:3, :3].sum() / 9
+    assert hs[1] <= Keff <= hs[4]
+
+
+def test_hs_bounds_two_phase_textbook_form(probe):
+    """Two phases: the classic two-phase bounds, K1 + c2 / (1/(K2-K1) +
+    3c1/(3K1+4G1)) and G1 + c2 / (1/(G2-G1) + 6c1(K1+2G1)/(5G1(3K1+4G1))),
+    with phase 1 the softer one; equal fractions of one phase give it back."""
+    K1, G1, K2, G2, c1 = 58.3e9, 26.9e9, 222.2e9, 166.7e9, 0.7
+    c2 = 1 - c1
+    b = _bounds(probe, [c1, c2], [K1, K2], [G1, G2], 0.5, 1.0, 0.5, 2.0)
+    klo = K1 + c2 / (1 / (K2 - K1) + 3 * c1 / (3 * K1 + 4 * G1))
+    khi = K2 + c1 / (1 / (K1 - K2) + 3 * c2 / (3 * K2 + 4 * G2))
+    glo = G1 + c2 / (1 / (G2 - G1) + 6 * c1 * (K1 + 2 * G1) / (5 * G1 * (3 * K1 + 4 * G1)))
+    ghi = G2 + c1 / (1 / (G1 - G2) + 6 * c2 * (K2 + 2 * G2) / (5 * G2 * (3 * K2 + 4 * G2)))
+    assert np.allclose(b[:2], [klo, glo], rtol=1e-13)
+    assert np.allclose(b[3:5], [khi, ghi], rtol=1e-13)
+    one = _bounds(probe, [1.0, 0.0], [K1, K2], [G1, G2], 1.0, 3.0, 0.0, 5.0)
+    assert np.allclose(one[:6], [K1, G1, 9 * K1 * G1 / (3 * K1 + G1)] * 2, rtol=1e-13)
+    assert one[6] == pytest.approx(3.0, rel=1e-14) and one[7] == pytest.approx(3.0, rel=1e-14)
+
+
+def test_conductivity_estimates_against_oracle_and_maxwell(probe):
+    """Mori-Tanaka / HS conductivity estimates at the true matrix fraction
+    equal the oracle's restatement of FFT_thermal3D.cpp:583-592 (its est[3:6]);
+    the matrix-referenced value is Maxwell's km (ki + 2km + 2f(ki-km)) /
+    (ki + 2km - f(ki-km)) for inclusion fraction f."""
+    n = 10
+    ids = gen.spheres(n, 2, 0.25, seed=9)
+    for km, ki in ((25.0, 30.0), (1.0, 1000.0), (40.0, 0.5)):
+        o = oracle.Thermal(n, n, n)
+        o.material(km, ki, ids)
+        r = o.homogenization(1e-4, 1e-4)
+        fm = float(np.count_nonzero(ids)) / ids.size
+        fi = 1 - fm
+        b = _bounds(probe, [1.0], [1.0], [1.0], fm, km, fi, ki)
+        mt, lo, up = r["est"][3:6]
+        assert b[6] == pytest.approx(mt, rel=1e-13) and b[6] == pytest.approx(lo, rel=1e-13)
+        assert b[7] == pytest.approx(up, rel=1e-13)
+        maxwell = km * (ki + 2 * km + 2 * fi * (ki - km)) / (ki + 2 * km - fi * (ki - km))
+        assert b[6] == pytest.approx(maxwell, rel=1e-13)
+        K = r["K"]
+        assert min(b[6], b[7]) <= np.trace(K) / 3 <= max(b[6], b[7])
+
+
 def test_vti_is_valid_image_data(probe, tmp_path):
     f = str(tmp_path / "a.vti")
     run(probe, "vti", f)
