@@ -53,6 +53,7 @@ def lib():
             "m3o_el_create": (vp, [ip, ip, ip, ip, ip]),
             "m3o_el_destroy": (None, [vp]),
             "m3o_el_set_cap": (None, [vp, ip]),
+            "m3o_el_set_threads": (None, [vp, ip, ip]),
             "m3o_el_set_scheme": (None, [vp, ip]),
             "m3o_el_material": (ip, [vp, dp, i32p, ip]),
             "m3o_el_set_load": (None, [vp, dp, dp]),
@@ -169,6 +170,10 @@ class Elastic:
 
     def set_cap(self, cap):
         lib().m3o_el_set_cap(self._h, int(cap))
+
+    def set_threads(self, fft_threads, loop_threads):
+        """threads of the FFTs and of the per-voxel loops (the reference: 3 and 1)"""
+        lib().m3o_el_set_threads(self._h, int(fft_threads), int(loop_threads))
 
     def set_scheme(self, basic):
         """1: basic scheme with the load held fixed (update eps <- eps_c)."""

@@ -469,6 +469,15 @@ void m3o_el_destroy(m3o_el *s)
 
 void m3o_el_set_cap(m3o_el *s, int cap) { s->cap = cap; }
 
+/* Threads of the FFTs and of the per-voxel loops, set apart: the reference
+ * runs FFTW with 3 threads (FFT_analysis3D.cpp:101-102) and its voxel loops
+ * serially, which (3, 1) reproduces for the CPU baseline. */
+void m3o_el_set_threads(m3o_el *s, int fft_threads, int loop_threads)
+{
+    s->plan.nthreads = fft_threads > 0 ? fft_threads : 1;
+    s->nthreads = loop_threads > 0 ? loop_threads : 1;
+}
+
 /* Basic Moulinec-Suquet scheme with the load held fixed (strain control): the
  * "e" loop with the update replaced by eps <- eps_c (sigma <- sigma_c in the
  * stress base), i.e. C_acc = I.  The reference runs its basic scheme only

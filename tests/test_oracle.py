@@ -9,11 +9,13 @@ the restatement (dropped term, wrong sign or index, transposed operand) fails:
 * full solver      -- homogeneous RVE (C_eff = C, 2 iterations), exact
                       laminate solutions (elastic and thermal), the
                       reference's own Hashin-Shtrikman bounds, the uniaxial
-                      J2 curve with linear hardening.
+                      J2 curve with linear hardening and with power-law
+                      hardening (n = 0.2, a scalar root by bisection).
 """
 import numpy as np
 import pytest
 
+import j2_model as j2m
 import oracle
 from mat3d_amd import gen
 
@@ -301,6 +303,28 @@ def test_j2_uniaxial_linear_hardening():
         assert abs(s0 - ref) < 1e-6 * max(abs(ref), sy), (eps, s0, ref)
         assert np.abs(row[2:7]).max() < 1e-6 * max(abs(s0), 1.0)  # uniaxial: other components vanish
     assert P[-1, 1] > sy  # the path went plastic
+
+
+@pytest.mark.parametrize("E,nu,sy,H,n_exp,emax", j2m.POWER_LAW_CASES)
+def test_j2_uniaxial_power_law(E, nu, sy, H, n_exp, emax):
+    """Homogeneous "ep" RVE with hardening exponent n = 0.2 under mixed
+    (uniaxial stress) control along tensor axis 0 against the scalar root of
+    the power-law flow rule (tests/j2_model.py).  An initial plastic strain
+    of 0 instead of EP0 moves the yield point by H EP0^0.2 (1.7% of H); a
+    Newton slip (wrong derivative, exponent n instead of n-1) moves every
+    plastic row."""
+    n = 4
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    mat = np.array([[E, nu, sy, H, n_exp, 0, 0]] * 2)
+    o = oracle.Elastic(n, n, n, "ep", nthreads=1)
+    o.material(mat, ids)
+    X0 = np.eye(6)[0]
+    o.set_load(emax * X0, X0)
+    assert o.run(1e-10, 1e-10, 20) > 0  # -1: the return map failed (NaN or no convergence)
+    P = o.prop_homogenized()
+    assert P.shape == (20, 9)
+    assert j2m.check_uniaxial_rows(P, E, sy, H, n_exp) >= 10
 
 
 def test_damage_uniaxial_closed_form():
