@@ -27,9 +27,19 @@ struct Comp6 {
 //   then per voxel the constitutive update (elastoPlastic :640-691 or
 //   elastoDamage :619-638) -> sigma; FP64 partial sums of sigma and eps
 //   (updateE :583-594); r2c of the 12 lines -> [sigma^ | eps^]
+// The FP64 ITER pass keeps eps, sigma and the return-map state (phi,
+// d_ep_p, sigma_eff) out of HBM: only the increment's last iterate needs
+// them (update_old_Variable :693-719, export_homogen_var :722-746), and FINAL
+// recomputes that iterate from the eps^ columns of S12, which the z and y
+// passes of its iteration left transformed along z and y (the x pass only
+// reads S12): an inverse y pass, then here the c2r and the constitutive
+// update (the history does not change within an increment).  The damage
+// variable is written every iteration: the next one compares with it.
 template <typename T, int BEH, int MODE, int LZ, int TY>
 __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 {
+    constexpr bool FIN = MODE == BMODE_FINAL;
+    constexpr bool STORE = !(MODE == BMODE_ITER && sizeof(T) == 8);
     constexpr int C = 12;
     constexpr int W = TY * C;
     constexpr int NZ = 2 * LZ;
@@ -54,13 +64,23 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     const uint8_t* ph = p.ph + vox0;
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
 
-    if (MODE == BMODE_ITER) {
-        const cx<T>* src = p.S6 + ((long long)x * g.Ny + y0) * NZH * 6;
-        for (int e = tid; e < TY * NZH * 6; e += NT) {
-            const int yl = e / (NZH * 6);
-            const int r = e - yl * (NZH * 6);
-            const int kz = r / 6, c = r - kz * 6;
-            sl[kz * W + yl * C + 6 + c] = src[e];
+    if (MODE == BMODE_ITER || FIN) {
+        if (FIN) {
+            const cx<T>* src = p.S12 + ((long long)x * g.Ny + y0) * NZH * 12;
+            for (int e = tid; e < TY * NZH * 6; e += NT) {
+                const int yl = e / (NZH * 6);
+                const int r = e - yl * (NZH * 6);
+                const int kz = r / 6, c = r - kz * 6;
+                sl[kz * W + yl * C + 6 + c] = src[((long long)yl * NZH + kz) * 12 + 6 + c];
+            }
+        } else {
+            const cx<T>* src = p.S6 + ((long long)x * g.Ny + y0) * NZH * 6;
+            for (int e = tid; e < TY * NZH * 6; e += NT) {
+                const int yl = e / (NZH * 6);
+                const int r = e - yl * (NZH * 6);
+                const int kz = r / 6, c = r - kz * 6;
+                sl[kz * W + yl * C + 6 + c] = src[e];
+            }
         }
         __syncthreads();
         // c2r pre-processing of the eps lines; kz = 0 and Nz/2 keep real parts
@@ -96,18 +116,20 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 #pragma unroll
     for (int q = 0; q < NBSUM; ++q) acc[q] = 0.0;
     bool bad = false;
+    // FINAL: the inverse y and z transforms of the unnormalised forward ones
+    const T fsc = FIN ? T(1.0 / ((double)g.Ny * (double)g.Nz)) : T(1);
     for (int q = tid; q < TY * LZ; q += NT) {
         const int yl = q / LZ, m = q - yl * LZ;
         const int v0 = yl * NZ + 2 * m;
         T e2[12], s2[12];
-        if (MODE == BMODE_ITER) {
+        if (MODE == BMODE_ITER || FIN) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
                 const cx<T> u = sl[m * W + yl * C + 6 + c];
-                e2[c] = u.re;
-                e2[6 + c] = u.im;
+                e2[c] = FIN ? u.re * fsc : u.re;
+                e2[6 + c] = FIN ? u.im * fsc : u.im;
             }
-            fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
+            if (STORE) fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
         } else {
             fast::vload<T, 12>(eps + (long long)v0 * 6, e2);
         }
@@ -122,9 +144,11 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
                 fast::vload<T, 6>(p.sig_old + vg * 6, so);
                 fast::vload<T, 6>(p.eps_old + vg * 6, eo);
                 if (!j2_update<T>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
-                p.phi[vg] = phi;
-                p.dep[vg] = dep;
-                p.seff[vg] = seff;
+                if (STORE) {
+                    p.phi[vg] = phi;
+                    p.dep[vg] = dep;
+                    p.seff[vg] = seff;
+                }
             } else {
                 T d = p.ep[vg];
                 damage_update<T>(mt, e, d, sg);
@@ -136,7 +160,8 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
                 acc[6 + c] += (double)e[c];
             }
         }
-        fast::vstore<T, 12>(sig + (long long)v0 * 6, s2);
+        if (STORE) fast::vstore<T, 12>(sig + (long long)v0 * 6, s2);
+        if (FIN) continue;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             sl[m * W + yl * C + c] = cx<T>{s2[c], s2[6 + c]};
@@ -144,6 +169,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         }
     }
     if (bad) ctl.err = 1;
+    if (FIN) return;
     // fixed-order block sums of the 12 means
     {
         const int lane = tid & 63, wid = tid >> 6;
@@ -347,9 +373,11 @@ void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (beh == BEH_EP) {
         if (mode == BMODE_START) z_mode<T, BEH_EP, BMODE_START>(g, p, st);
+        else if (mode == BMODE_FINAL) z_mode<T, BEH_EP, BMODE_FINAL>(g, p, st);
         else z_mode<T, BEH_EP, BMODE_ITER>(g, p, st);
     } else {
         if (mode == BMODE_START) z_mode<T, BEH_ED, BMODE_START>(g, p, st);
+        else if (mode == BMODE_FINAL) z_mode<T, BEH_ED, BMODE_FINAL>(g, p, st);
         else z_mode<T, BEH_ED, BMODE_ITER>(g, p, st);
     }
 }
@@ -358,6 +386,12 @@ template <typename T>
 void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st)
 {
     y_run<T, Comp12, -1, BUF_K>(g, p, st);
+}
+
+template <typename T>
+void launch_y12_inv(const Geo& g, const Ptr<T>& p, hipStream_t st)
+{
+    y_run<T, Comp12, 1, BUF_K>(g, p, st);
 }
 
 template <typename T>
@@ -378,6 +412,7 @@ template void launch_z<double>(int, int, const Geo&, const BPtr<double>&, hipStr
 template void launch_z<float>(int, int, const Geo&, const BPtr<float>&, hipStream_t);
 template void launch_y12<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y12<float>(const Geo&, const Ptr<float>&, hipStream_t);
+template void launch_y12_inv<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y6<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y6<float>(const Geo&, const Ptr<float>&, hipStream_t);
 template void launch_x<double>(const Geo&, const BPtr<double>&, hipStream_t);

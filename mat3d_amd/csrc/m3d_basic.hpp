@@ -26,7 +26,10 @@
 namespace m3d {
 
 enum { BEH_EP = 1, BEH_ED = 2 };
-enum { BMODE_START = 0, BMODE_ITER = 1, BMODE_RESTART = 2 };
+// FINAL (specialised kernels only): the fields and state of the last iterate
+// of an increment, recomputed from the [sigma^ | eps^] spectrum its z and y
+// passes left behind (the FP64 iterations do not write them)
+enum { BMODE_START = 0, BMODE_ITER = 1, BMODE_RESTART = 2, BMODE_FINAL = 3 };
 constexpr int NBSUM = 12; // sums per solve in the z pass: sigma_c, eps_c
 
 template <typename T>
@@ -466,8 +469,30 @@ __device__ __forceinline__ void update_E(Ctl& c, const double* sbar, const doubl
 // MODE_RESTART: the iterate itNo recomputed in FP64 from its FP32 strain
 //   (end of an FP32 phase): sum_field_old and E as in MODE_ITER, itNo kept;
 //   the check after it records eqVar without a stop decision
+//
+// The z pass leaves NBSUM partial sums per workgroup (nbz of them per sum,
+// 18 432 at 192^3).  k_bsum_parts adds them in BSPLIT contiguous shares, one
+// workgroup per (share, sum), and k_bfin1 adds the BSPLIT shares: a fixed
+// summation order, so the result does not depend on timing.
+constexpr int BSPLIT = 64;
+
+template <int U = 0> // a template so that every unit including this header may instantiate it
+__global__ void __launch_bounds__(256) k_bsum_parts(Geo g, const Ctl* ctl, const double* psum, double* part)
+{
+    __shared__ double red[256 / 64];
+    const int s = blockIdx.z, q = blockIdx.y, k = blockIdx.x;
+    if (ctl[s].done) return;
+    const int per = (g.nbz + BSPLIT - 1) / BSPLIT;
+    const int b = k * per, e = min(g.nbz, b + per);
+    const double* a = psum + ((long long)s * NBSUM + q) * g.nbz;
+    double v = 0;
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) v += a[i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) part[((long long)s * NBSUM + q) * BSPLIT + k] = v;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const double* psum)
+__global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const double* part)
 {
     __shared__ double red[FTHREADS / 64];
     const int s = blockIdx.x;
@@ -476,7 +501,7 @@ __global__ void __launch_bounds__(FTHREADS) k_bfin1(Geo g, Ctl* ctl, const doubl
     __syncthreads();
     if (done) return;
     double m[NBSUM];
-    for (int q = 0; q < NBSUM; ++q) m[q] = ordered_sum(psum + ((long long)s * NBSUM + q) * g.nbz, g.nbz, red);
+    for (int q = 0; q < NBSUM; ++q) m[q] = ordered_sum(part + ((long long)s * NBSUM + q) * BSPLIT, BSPLIT, red);
     if (threadIdx.x == 0) {
         double sbar[6], ebar[6], S = 0;
         for (int q = 0; q < 6; ++q) {

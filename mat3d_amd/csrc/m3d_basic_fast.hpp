@@ -48,6 +48,9 @@ template <typename T>
 void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st);
 template <typename T>
 void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st);
+// inverse y pass of the 12-component spectrum (FP64), ahead of the FINAL z pass
+template <typename T>
+void launch_y12_inv(const Geo& g, const Ptr<T>& p, hipStream_t st);
 template <typename T>
 void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st);
 template <typename T>

@@ -30,6 +30,7 @@ struct BasicRunner {
     unsigned char* S6s = nullptr;
     unsigned char* S12s = nullptr;
     double* psum = nullptr;
+    double* psum2 = nullptr; // [NBSUM][BSPLIT] shares of psum
     double* pchk = nullptr;
     double* bs = nullptr;
     double* mat = nullptr;
@@ -78,6 +79,7 @@ struct BasicRunner {
             e.alloc(&S6, sizeof(cx<double>) * 12 * H);
             e.alloc(&S12, sizeof(cx<double>) * 12 * H);
             e.alloc(&psum, sizeof(double) * NBSUM * nbz);
+            e.alloc(&psum2, sizeof(double) * NBSUM * BSPLIT);
             e.alloc(&pchk, sizeof(double) * (nbx + 1));
             e.alloc(&bs, sizeof(double) * 14 * nblk);
             e.alloc(&mat, sizeof(double) * 256 * 8);
@@ -96,10 +98,10 @@ struct BasicRunner {
     }
     void release()
     {
-        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)psum, (void*)pchk, (void*)bs, (void*)mat,
+        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)psum, (void*)psum2, (void*)pchk, (void*)bs, (void*)mat,
                         (void*)ctl, (void*)f6s, (void*)f1s, (void*)S6s, (void*)S12s})
             if (q) (void)hipFree(q);
-        f6 = f1 = psum = pchk = bs = mat = nullptr;
+        f6 = f1 = psum = psum2 = pchk = bs = mat = nullptr;
         S6 = S12 = nullptr;
         f6s = f1s = nullptr;
         S6s = S12s = nullptr;
@@ -157,6 +159,15 @@ struct BasicRunner {
     BPtr<double> bptr() { return bptr_t<double>(); }
     Ptr<double> yptr() { return yptr_t<double>(); }
 
+    // the per-solve sums of the z pass (two launches, fixed order) and the
+    // increment/iteration bookkeeping that follows them
+    template <int MODE>
+    void bfin1(const Geo& g)
+    {
+        hipLaunchKernelGGL(k_bsum_parts<0>, dim3(BSPLIT, NBSUM, 1), dim3(256), 0, e.st, g, ctl, psum, psum2);
+        hipLaunchKernelGGL((k_bfin1<MODE>), dim3(1), dim3(FTHREADS), 0, e.st, g, ctl, psum2);
+    }
+
     template <bool GEN>
     void stage(int mode)
     {
@@ -172,13 +183,13 @@ struct BasicRunner {
                 hipLaunchKernelGGL((k_zbasic<double, BEH_EP, BMODE_START, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
             else
                 hipLaunchKernelGGL((k_zbasic<double, BEH_ED, BMODE_START, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
-            hipLaunchKernelGGL((k_bfin1<BMODE_START>), dim3(1), dim3(FTHREADS), 0, st, g12, ctl, psum);
+            bfin1<BMODE_START>(g12);
         } else {
             if (beh == BEH_EP)
                 hipLaunchKernelGGL((k_zbasic<double, BEH_EP, BMODE_ITER, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
             else
                 hipLaunchKernelGGL((k_zbasic<double, BEH_ED, BMODE_ITER, GEN>), dim3(g12.nbz), dim3(ZTHREADS), zsm, st, g12, bp);
-            hipLaunchKernelGGL((k_bfin1<BMODE_ITER>), dim3(1), dim3(FTHREADS), 0, st, g12, ctl, psum);
+            bfin1<BMODE_ITER>(g12);
         }
         hipLaunchKernelGGL((k_ypass<double, 12, -1, BUF_K, GEN>), dim3(g12.Nx * g12.nky), dim3(XYTHREADS), ysm12, st, g12, yp);
         hipLaunchKernelGGL((k_xbasic<double, GEN>), dim3(g12.nbx), dim3(XYTHREADS), xsm, st, g12, bp);
@@ -195,14 +206,36 @@ struct BasicRunner {
         BPtr<T> bp = bptr_t<T>();
         Ptr<T> yp = yptr_t<T>();
         bfast::launch_z<T>(beh, mode == BMODE_ITER ? BMODE_ITER : BMODE_START, gf, bp, st);
-        if (mode == BMODE_START) hipLaunchKernelGGL((k_bfin1<BMODE_START>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
+        if (mode == BMODE_START) bfin1<BMODE_START>(gf);
         else if (mode == BMODE_RESTART)
-            hipLaunchKernelGGL((k_bfin1<BMODE_RESTART>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
-        else hipLaunchKernelGGL((k_bfin1<BMODE_ITER>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, psum);
+            bfin1<BMODE_RESTART>(gf);
+        else bfin1<BMODE_ITER>(gf);
         bfast::launch_y12<T>(gf, yp, st);
         bfast::launch_x<T>(gf, bp, st);
         hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, pchk);
         bfast::launch_y6<T>(gf6, yp, st);
+    }
+
+    // the fields and state of the increment's last iterate on the
+    // specialised FP64 path, whose iterations do not store them: inverse y
+    // pass of the [sigma^ | eps^] spectrum, then the z pass in FINAL mode.
+    // Both passes skip a solve marked done, so the flag is cleared around
+    // them; c is the host copy of the control block after the iterations.
+    void final_fast(Ctl& c)
+    {
+        hipStream_t st = e.st;
+        Ctl run = c;
+        run.done = 0;
+        HCHK(hipMemcpyAsync(ctl, &run, sizeof run, hipMemcpyHostToDevice, st));
+        bfast::launch_y12_inv<double>(gf, yptr_t<double>(), st);
+        bfast::launch_z<double>(beh, BMODE_FINAL, gf, bptr_t<double>(), st);
+        HCHK(hipGetLastError());
+        Ctl after;
+        HCHK(hipMemcpyAsync(&after, ctl, sizeof after, hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+        if (after.err) c.done = 3;
+        HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+        HCHK(hipStreamSynchronize(st));
     }
 
     void run_stage(int mode)
@@ -321,6 +354,7 @@ struct BasicRunner {
                     stage_fast<double>(BMODE_RESTART);
                     HCHK(hipGetLastError());
                     replay(g64.x);
+                    if (c.done != 3) final_fast(c);
                     fp64_run = false;
                 } else {
                     // the FP32 return map failed: redo the increment in FP64 (its
@@ -337,6 +371,7 @@ struct BasicRunner {
                 run_stage(BMODE_START);
                 HCHK(hipGetLastError());
                 replay(g64.x);
+                if (fast && c.done != 3) final_fast(c);
             }
             if (c.done == 3)
                 throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};

@@ -7,6 +7,7 @@ import os
 import numpy as np
 import pytest
 
+import j2_model as j2m
 import mat3d_amd as m
 import oracle
 from mat3d_amd import gen
@@ -157,6 +158,77 @@ def test_j2_uniaxial_closed_form_gpu():
     for eps, s0 in s.prop_homogenized[:, :2]:
         ref = E * eps if s0 <= sy else (eps + sy / H) / (1 / E + 1 / H)
         assert abs(s0 - ref) < 1e-6 * max(abs(ref), sy)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32)])
+def test_j2_power_law_matches_oracle(shape):
+    """The reference CLI's material table (example/3d/main.cpp:107-108):
+    matrix with power-law hardening n = 0.2, inclusions with linear
+    hardening, both yielding.  16^3 runs the runtime-planned basic kernels,
+    32^3 the specialised ones; both run the Newton branch of the return map."""
+    n = shape[0]
+    ids = gen.spheres(n, 3, 0.2, seed=11)
+    X0 = np.eye(6)[1]
+    (P_o, e_o, s_o, it_o), (P, e, s, info) = run_both(shape, "ep", j2m.MAT_REF_TABLE, ids, 0.02 * X0, X0, 6)
+    print(f"[j2 n=0.2 {shape}] iters oracle {it_o} gpu {info['iters_fp64']} last row {P[-1, :3]}")
+    assert it_o > 0
+    assert info["iters_fp64"] == it_o
+    assert P.shape == P_o.shape
+    # axial stress (load along tensor axis 1: column 2): the macroscopic tangent has dropped, both phases yield
+    assert P[-1, 2] - P[-2, 2] < 0.5 * (P[1, 2] - P[0, 2])
+    assert rel(P, P_o) < 1e-9
+    assert rel(s, s_o) < 1e-9 and rel(e, e_o) < 1e-9
+
+
+@pytest.mark.parametrize("n,case", [(32, 0), (32, 1), (192, 0), (192, 1)])
+def test_j2_uniaxial_power_law_gpu(n, case):
+    """Homogeneous "ep" RVE, hardening exponent n = 0.2, uniaxial stress
+    control: every prop_homogenized row against the scalar root of the flow
+    rule (tests/j2_model.py).  192^3 is the elastoplastic workload's grid,
+    on its specialised basic-scheme kernels."""
+    E, nu, sy, H, nexp, emax = j2m.POWER_LAW_CASES[case]
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    mat = np.array([[E, nu, sy, H, nexp, 0, 0]] * 2)
+    s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
+    s.material(mat, ids)
+    X0 = np.eye(6)[0]
+    s.setLoad(emax * X0, X0)
+    s.runAnalysis(1e-10, 1e-10, 20)
+    assert s.prop_homogenized.shape == (20, 9)
+    assert j2m.check_uniaxial_rows(s.prop_homogenized, E, sy, H, nexp) >= 10
+
+
+def test_j2_uniaxial_linear_192():
+    """Linear hardening (n = 1) at 192^3, the elastoplastic workload's grid:
+    sigma = E eps below yield, (eps + sy/H) / (1/E + 1/H) above."""
+    n = 192
+    ids = np.ones((n, n, n), dtype=np.int32)
+    ids[0, 0, 0] = 0
+    E, nu, sy, H = 70e9, 0.3, 100e6, 1e9
+    mat = np.array([[E, nu, sy, H, 1, 0, 0], [E, nu, sy, H, 1, 0, 0]])
+    s = m.FFT_Analysis3D(n, n, n, "ep", precision="fp64")
+    s.material(mat, ids)
+    X0 = np.eye(6)[0]
+    s.setLoad(0.004 * X0, X0)
+    s.runAnalysis(1e-10, 1e-10, 20)
+    assert j2m.check_uniaxial_rows(s.prop_homogenized, E, sy, H, 1.0) >= 10
+
+
+def test_j2_c4_microstructure_matches_oracle():
+    """The elastoplastic workload's first ten increments (spheres of radius 8
+    at 25%, E = 0.02 e_1 over 50 increments: 0.004 e_1 over 10) on a 48^3
+    grid against the oracle: same iterations, prop_homogenized and fields
+    within 1e-9."""
+    n = 48
+    ids = gen.spheres(n, 8, 0.25, seed=4)
+    X0 = np.eye(6)[0]
+    (P_o, e_o, s_o, it_o), (P, e, s, info) = run_both((n, n, n), "ep", gen.MAT_C4, ids, 0.004 * X0, X0, 10)
+    print(f"[j2 C4 48^3] iters oracle {it_o} gpu {info['iters_fp64']} last row {P[-1, :3]}")
+    assert info["iters_fp64"] == it_o
+    assert P[-1, 1] - P[-2, 1] < 0.5 * (P[1, 1] - P[0, 1])  # the macroscopic tangent has dropped: yielding
+    assert rel(P, P_o) < 1e-9
+    assert rel(s, s_o) < 1e-9 and rel(e, e_o) < 1e-9
 
 
 def test_damage_uniaxial_closed_form_gpu():
