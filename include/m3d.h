@@ -141,11 +141,25 @@ int m3d_elastic_set_option(m3d_elastic *h, const char *key, double value);
 int m3d_elastic_profile(m3d_elastic *h, int precision, int reps, double *ms6, int *geom9);
 
 /* Variants of the FP32 x/y passes the autotuner chose for the last solve's
- * shape (no reference counterpart): threads per workgroup of the x pass with
- * the Green step, its wide kz tile (1) or not (0), threads of the y pass.
+ * shape (no reference counterpart; it stands where the reference plans its
+ * FFTs, FFT_analysis3D.cpp:101-150): threads per workgroup of the x pass with
+ * the Green step, its wide kz tile (1) or not (0), threads of the y pass, the
+ * radix plan of each (0: radix 16 first, 1: radix 8 first) and whether the
+ * choice came from the tuning file (1) or was timed in this process (0).
  * Option "autotune" (default 1) times the variants at the first FP32 solve
- * of a (grid, solves) shape. */
-int m3d_elastic_get_tuning(m3d_elastic *h, int *xg_threads, int *xg_wide, int *y_threads);
+ * of a (device, grid, solves) shape; every pointer may be NULL. */
+int m3d_elastic_get_tuning(m3d_elastic *h, int *xg_threads, int *xg_wide, int *y_threads, int *xg_radix, int *y_radix,
+                           int *from_file);
+
+/* Every tuning choice this process timed or read from the tuning file, one
+ * line each, into buf (len bytes, NUL-terminated, truncated to fit; *needed =
+ * bytes of the whole report).  No reference counterpart. */
+int m3d_tuning_report(char *buf, int len, int *needed);
+
+/* Path of the tuning file: $M3D_TUNE_FILE, else $XDG_CACHE_HOME or
+ * $HOME/.cache, then /mat3d_amd/tune.txt; "" when M3D_TUNE_FILE is "none"
+ * (choices kept in the process only).  No reference counterpart. */
+const char *m3d_tuning_file(void);
 
 const char *m3d_elastic_last_error(m3d_elastic *h);
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "m3d_elastic_create", "m3d_elastic_destroy", "m3d_elastic_material", "m3d_elastic_set_load",
     "m3d_elastic_run", "m3d_elastic_homogenization", "m3d_elastic_get_fields", "m3d_elastic_info",
     "m3d_elastic_set_option", "m3d_elastic_last_error", "m3d_elastic_profile", "m3d_elastic_get_prop", "m3d_elastic_get_state",
-    "m3d_elastic_get_tuning", "m3d_device_copy",
+    "m3d_elastic_get_tuning", "m3d_tuning_report", "m3d_tuning_file", "m3d_device_copy",
     "m3d_thermal_create", "m3d_thermal_destroy", "m3d_thermal_material", "m3d_thermal_material_n",
     "m3d_thermal_set_load", "m3d_thermal_run", "m3d_thermal_homogenization", "m3d_thermal_get_fields",
     "m3d_thermal_set_option", "m3d_thermal_last_error",
@@ -49,6 +49,16 @@ EXPORTS = [
     "m3d_perc_create", "m3d_perc_destroy", "m3d_perc_last_error", "m3d_perc_run", "m3d_perc_result",
     "m3d_perc_surface", "m3d_perc_set_timing", "m3d_perc_timings",
 ]
+
+
+def tuning_report():
+    """Every autotuner choice this process timed or read from the tuning file
+    (one line each) and the tuning file's path."""
+    need = ct.c_int(0)
+    lib().m3d_tuning_report(None, 0, ct.byref(need))
+    buf = ct.create_string_buffer(max(1, need.value))
+    lib().m3d_tuning_report(buf, len(buf), ct.byref(need))
+    return {"file": lib().m3d_tuning_file().decode(), "choices": [l for l in buf.value.decode().splitlines() if l]}
 
 
 class RunInfo(ct.Structure):
@@ -96,7 +106,9 @@ def lib():
             "m3d_elastic_profile": (ip, [vp, ip, ip, dp, ipp]),
             "m3d_elastic_get_prop": (ip, [vp, dp, ipp]),
             "m3d_elastic_get_state": (ip, [vp, dp, dp]),
-            "m3d_elastic_get_tuning": (ip, [vp, ipp, ipp, ipp]),
+            "m3d_elastic_get_tuning": (ip, [vp, ipp, ipp, ipp, ipp, ipp, ipp]),
+            "m3d_tuning_report": (ip, [ct.c_char_p, ip, ipp]),
+            "m3d_tuning_file": (cp, []),
             "m3d_device_copy": (ip, [ct.c_double, ip, dp]),
             "m3d_rec_create": (ip, [dp, ip, ip, ip, ct.POINTER(vp)]),
             "m3d_rec_destroy": (ip, [vp]),
@@ -271,9 +283,10 @@ class FFT_Analysis3D:
 
     def tuning(self):
         """Autotuned variants of the FP32 x/y passes for the last solve's shape."""
-        a, b, c = ct.c_int(0), ct.c_int(0), ct.c_int(0)
-        self._chk(lib().m3d_elastic_get_tuning(self._h, ct.byref(a), ct.byref(b), ct.byref(c)))
-        return {"xgreen_threads": a.value, "xgreen_wide_tile": b.value, "ypass_threads": c.value}
+        v = [ct.c_int(0) for _ in range(6)]
+        self._chk(lib().m3d_elastic_get_tuning(self._h, *(ct.byref(x) for x in v)))
+        return {"xgreen_threads": v[0].value, "xgreen_wide_tile": v[1].value, "ypass_threads": v[2].value,
+                "xgreen_radix_plan": v[3].value, "ypass_radix_plan": v[4].value, "from_tuning_file": v[5].value}
 
     def profile_kernels(self, precision="fp32_polish", reps=10):
         """Average ms per launch of the per-iteration kernels (HIP events on the

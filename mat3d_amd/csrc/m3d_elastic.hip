@@ -3,6 +3,10 @@
 #include "m3d_basic_host.hpp"
 #include "m3d_elastic_props.hpp"
 
+#include <chrono>
+#include <exception>
+#include <thread>
+
 // ====================================================================== ABI
 struct m3d_elastic {
     Engine<Elastic>* eng = nullptr;
@@ -288,14 +292,51 @@ int m3d_elastic_get_state(m3d_elastic* h, double* ep_eff_or_d, double* sig_eff)
     });
 }
 
-int m3d_elastic_get_tuning(m3d_elastic* h, int* xg_threads, int* xg_wide, int* y_threads)
+int m3d_elastic_get_tuning(m3d_elastic* h, int* xg_threads, int* xg_wide, int* y_threads, int* xg_radix, int* y_radix,
+                           int* from_file)
 {
     return guarded(h, [&]() {
-        if (xg_threads) *xg_threads = h->eng->tune32.xg_nt;
-        if (xg_wide) *xg_wide = h->eng->tune32.xg_wide;
-        if (y_threads) *y_threads = h->eng->tune32.y_nt;
+        const XyTune& t = h->eng->tune32;
+        if (xg_threads) *xg_threads = t.xg_nt;
+        if (xg_wide) *xg_wide = t.xg_wide;
+        if (y_threads) *y_threads = t.y_nt;
+        if (xg_radix) *xg_radix = t.xg_rp;
+        if (y_radix) *y_radix = t.y_rp;
+        if (from_file) *from_file = t.from_file;
         return M3D_OK;
     });
+}
+
+int m3d_tuning_report(char* buf, int len, int* needed)
+{
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        for (const auto& kv : tune_cache()) {
+            const TuneKey& k = kv.first;
+            const XyTune& t = kv.second;
+            char line[256];
+            std::snprintf(line, sizeof line, "%s %dx%dx%d c%d solves %d: xgreen %d threads, %s kz tile, radix plan %d; "
+                                             "ypass %d threads, radix plan %d (%s)\n",
+                          k.dev.c_str(), k.nx, k.ny, k.nz, k.c, k.ns, t.xg_nt, t.xg_wide ? "wide" : "default", t.xg_rp,
+                          t.y_nt, t.y_rp, t.from_file ? "tuning file" : "timed");
+            out += line;
+        }
+    }
+    if (needed) *needed = (int)out.size() + 1;
+    if (buf && len > 0) {
+        const size_t n = std::min((size_t)len - 1, out.size());
+        std::memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
+    return M3D_OK;
+}
+
+const char* m3d_tuning_file(void)
+{
+    static std::string p;
+    p = tune_file_path();
+    return p.c_str();
 }
 
 int m3d_elastic_profile(m3d_elastic* h, int precision, int reps, double* ms6, int* geom9)
@@ -418,7 +459,83 @@ int m3d_batch_add(m3d_batch* b, int nx, int ny, int nz, const double* mat, const
 
 // homogenization() of every RVE, FFT_analysis3D.cpp:856-999: problems of one
 // grid shape share an engine; their 6 load cases each run as one batch of
-// independent solves (sub-batched to fit the device memory)
+// independent solves.  When every shape group fits the device memory in one
+// device batch, the groups' engines run at the same time, one host thread
+// and one stream each (allocation and autotuning done first, one engine
+// after the other), so small grids and the host's polling between graph
+// replays of one group overlap the others' kernels; otherwise the groups run
+// one after another, sub-batched to fit.
+namespace {
+
+struct GroupRun {
+    std::vector<std::array<double, 6>> loads;
+    std::vector<double> norms;
+    std::vector<int> sbv, rvv;
+    std::vector<double> means;
+    std::exception_ptr err;
+};
+
+void batch_group_setup(m3d_batch* b, BatchGroup& G)
+{
+    Engine<Elastic>& e = *G.eng;
+    e.polish_factor = b->polish_factor;
+    e.fp32_floor = b->fp32_floor;
+    e.rollback = b->rollback;
+    e.cap = b->cap;
+    e.chunk = b->chunk;
+    e.refine = b->refine;
+    e.autotune = b->autotune;
+}
+
+// loads, norms, base flags and RVE indices of members [k0, k1) of G; uploads
+// their phase ids and coefficient tables unless resident
+void batch_group_load(m3d_batch* b, BatchGroup& G, size_t k0, size_t k1, GroupRun& run)
+{
+    Engine<Elastic>& e = *G.eng;
+    const std::vector<int>& members = G.members;
+    const int R = (int)(k1 - k0);
+    run.loads.clear();
+    run.norms.clear();
+    run.sbv.clear();
+    run.rvv.clear();
+    for (int r = 0; r < R; ++r) {
+        const BatchProblem& q = b->probs[members[k0 + r]];
+        elastic_unit_loads(q.P, e.N, run.loads, run.norms);
+        for (int w = 0; w < 6; ++w) {
+            run.sbv.push_back(q.P.strain_base);
+            run.rvv.push_back(r);
+        }
+    }
+    if (!G.loaded) {
+        std::vector<uint8_t> ids((size_t)R * e.N);
+        std::vector<double> coef((size_t)R * 256 * NCOEF), prm((size_t)2 * R);
+        for (int r = 0; r < R; ++r) {
+            const BatchProblem& q = b->probs[members[k0 + r]];
+            std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
+            elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF, b->scheme == 1);
+            prm[2 * r] = q.P.la0;
+            prm[2 * r + 1] = q.P.mu0;
+        }
+        e.upload_material(ids, coef, prm);
+        G.loaded = (R == (int)members.size()); // one device batch: keep it resident
+    }
+}
+
+void batch_group_store(m3d_batch* b, BatchGroup& G, size_t k0, size_t k1, const std::vector<double>& m)
+{
+    Engine<Elastic>& e = *G.eng;
+    for (int r = 0; r < (int)(k1 - k0); ++r) {
+        BatchProblem& q = b->probs[G.members[k0 + r]];
+        elastic_assemble(q.P.strain_base, m.data() + (size_t)r * 72, q.C, q.S, q.ep, q.sg, &q.ret);
+        for (int w = 0; w < 6; ++w) {
+            q.info[w] = e.info[(size_t)6 * r + w];
+            b->voxel_its += (double)e.N * q.info[w].iterations;
+        }
+    }
+}
+
+} // namespace
+
 int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
 {
     return guarded(b, [&]() {
@@ -440,63 +557,68 @@ int m3d_batch_homogenization(m3d_batch* b, double tol_field, double tol_eq)
             }
             b->grouped = true;
         }
+        // engines; device bytes the groups still need to run as one device batch each
+        size_t extra = 0;
         for (BatchGroup& G : b->groups) {
-            if (!G.eng) {
-                G.eng = new Engine<Elastic>(G.shape[0], G.shape[1], G.shape[2], b->precision);
-                // RVEs per device batch: 70% of free memory, or the max_solves option
-                size_t fr = 0, tot = 0;
-                HCHK(hipMemGetInfo(&fr, &tot));
-                G.per = (long long)(0.7 * (double)fr / (double)(6 * G.eng->bytes_per_solve() + (size_t)G.eng->N));
-                if (b->max_solves > 0) G.per = std::min<long long>(G.per, b->max_solves / 6);
-                if (G.per < 1) throw ArgError{M3D_ERR_OOM, "batch_homogenization: one RVE's 6 solves do not fit"};
+            if (!G.eng) G.eng = new Engine<Elastic>(G.shape[0], G.shape[1], G.shape[2], b->precision);
+            batch_group_setup(b, G);
+            const int ns = 6 * (int)G.members.size();
+            if (G.eng->cap_solves < ns || (b->precision == M3D_FP32_POLISH && G.eng->ring32 != G.eng->ring_needed()))
+                extra += (size_t)ns * G.eng->bytes_per_solve() + (size_t)G.members.size() * G.eng->N;
+        }
+        size_t fr = 0, tot = 0;
+        HCHK(hipMemGetInfo(&fr, &tot));
+        const bool together = b->max_solves == 0 && b->groups.size() > 1 && (double)extra <= 0.7 * (double)fr;
+        if (together) {
+            std::vector<GroupRun> runs(b->groups.size());
+            for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+                BatchGroup& G = b->groups[gi];
+                G.per = (long long)G.members.size();
+                batch_group_load(b, G, 0, G.members.size(), runs[gi]);
+                G.eng->prepare(6 * (int)G.members.size());
             }
-            Engine<Elastic>& e = *G.eng;
-            e.polish_factor = b->polish_factor;
-            e.fp32_floor = b->fp32_floor;
-            e.rollback = b->rollback;
-            e.cap = b->cap;
-            e.chunk = b->chunk;
-            e.refine = b->refine;
-            e.autotune = b->autotune;
-            const std::vector<int>& members = G.members;
-            for (size_t k0 = 0; k0 < members.size(); k0 += (size_t)G.per) {
-                const size_t k1 = std::min(members.size(), k0 + (size_t)G.per);
-                const int R = (int)(k1 - k0);
-                std::vector<std::array<double, 6>> loads;
-                std::vector<double> norms;
-                std::vector<int> sbv, rvv;
-                for (int r = 0; r < R; ++r) {
-                    const BatchProblem& q = b->probs[members[k0 + r]];
-                    elastic_unit_loads(q.P, e.N, loads, norms);
-                    for (int w = 0; w < 6; ++w) {
-                        sbv.push_back(q.P.strain_base);
-                        rvv.push_back(r);
+            int dev = 0;
+            HCHK(hipGetDevice(&dev));
+            const auto t0 = std::chrono::steady_clock::now();
+            std::vector<std::thread> th;
+            th.reserve(b->groups.size());
+            for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+                th.emplace_back([&, gi]() {
+                    GroupRun& run = runs[gi];
+                    try {
+                        HCHK(hipSetDevice(dev));
+                        Engine<Elastic>& e = *b->groups[gi].eng;
+                        e.solve(run.loads, tol_field, tol_eq, run.norms, 1, run.sbv, run.rvv);
+                        run.means = e.means();
+                    } catch (...) {
+                        run.err = std::current_exception();
                     }
+                });
+            }
+            for (std::thread& t : th) t.join();
+            b->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            for (GroupRun& run : runs)
+                if (run.err) std::rethrow_exception(run.err);
+            for (size_t gi = 0; gi < b->groups.size(); ++gi)
+                batch_group_store(b, b->groups[gi], 0, b->groups[gi].members.size(), runs[gi].means);
+        } else {
+            for (BatchGroup& G : b->groups) {
+                Engine<Elastic>& e = *G.eng;
+                if (G.per < 1) {
+                    // RVEs per device batch: 70% of free memory, or the max_solves option
+                    HCHK(hipMemGetInfo(&fr, &tot));
+                    G.per = (long long)(0.7 * (double)fr / (double)(6 * e.bytes_per_solve() + (size_t)e.N));
+                    if (b->max_solves > 0) G.per = std::min<long long>(G.per, b->max_solves / 6);
+                    if (G.per < 1) throw ArgError{M3D_ERR_OOM, "batch_homogenization: one RVE's 6 solves do not fit"};
                 }
-                if (!G.loaded) {
-                    std::vector<uint8_t> ids((size_t)R * e.N);
-                    std::vector<double> coef((size_t)R * 256 * NCOEF), prm((size_t)2 * R);
-                    for (int r = 0; r < R; ++r) {
-                        const BatchProblem& q = b->probs[members[k0 + r]];
-                        std::memcpy(ids.data() + (size_t)r * e.N, q.ids.data(), (size_t)e.N);
-                        elastic_tables(q.P, coef.data() + (size_t)r * 256 * NCOEF, b->scheme == 1);
-                        prm[2 * r] = q.P.la0;
-                        prm[2 * r + 1] = q.P.mu0;
-                    }
-                    e.upload_material(ids, coef, prm);
-                    G.loaded = (R == (int)members.size()); // one device batch: keep it resident
+                for (size_t k0 = 0; k0 < G.members.size(); k0 += (size_t)G.per) {
+                    const size_t k1 = std::min(G.members.size(), k0 + (size_t)G.per);
+                    GroupRun run;
+                    batch_group_load(b, G, k0, k1, run);
+                    e.solve(run.loads, tol_field, tol_eq, run.norms, 1, run.sbv, run.rvv);
+                    batch_group_store(b, G, k0, k1, e.means());
+                    b->ms += e.info.empty() ? 0.0 : e.info[0].time_ms;
                 }
-                e.solve(loads, tol_field, tol_eq, norms, 1, sbv, rvv);
-                const std::vector<double> m = e.means();
-                for (int r = 0; r < R; ++r) {
-                    BatchProblem& q = b->probs[members[k0 + r]];
-                    elastic_assemble(q.P.strain_base, m.data() + (size_t)r * 72, q.C, q.S, q.ep, q.sg, &q.ret);
-                    for (int w = 0; w < 6; ++w) {
-                        q.info[w] = e.info[(size_t)6 * r + w];
-                        b->voxel_its += (double)e.N * q.info[w].iterations;
-                    }
-                }
-                b->ms += e.info.empty() ? 0.0 : e.info[0].time_ms;
             }
         }
         b->ran = true;

@@ -16,9 +16,13 @@
 #include <tuple>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "../../include/m3d.h"
 #include "m3d_kernels.hpp"
@@ -188,17 +192,25 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
     // spectrum row layout (the runtime-planned kernels address rows (x, y)
     // at (x * Ny + y) * Nzh * C: plain rows without padding)
     const long long rowc = (long long)g.Nzh * C;
+    const bool all_fast = g.fz && g.fy && g.fx;
+    // the specialised kernels start every row on a 64-byte boundary (whole kz
+    // slots of padding): their y and x passes move a row's kz tile as one
+    // contiguous chunk, and a chunk that straddles 64-byte sectors is written
+    // as partial sectors (FP32, C = 6, 256^3: rows of 129 slots, 6192 bytes,
+    // padded to 132 slots)
+    long long rowp = rowc;
+    if (all_fast)
+        while ((rowp * (long long)csz) % 64) rowp += C;
     g.lxb = 0;
     g.xbm = 0;
-    g.sxb = rowc;
-    g.sy = rowc;
-    g.sxa = (long long)Ny * rowc;
-    const bool all_fast = g.fz && g.fy && g.fx;
+    g.sxb = rowp;
+    g.sy = rowp;
+    g.sxa = (long long)Ny * rowp;
     if (all_fast && M3D_XBLOCK > 1 && Nx % M3D_XBLOCK == 0) {
         while ((1 << g.lxb) < M3D_XBLOCK) ++g.lxb;
         g.xbm = M3D_XBLOCK - 1;
-        g.sxa = (long long)Ny * M3D_XBLOCK * rowc;
-        g.sy = (long long)M3D_XBLOCK * rowc;
+        g.sxa = (long long)Ny * M3D_XBLOCK * rowp;
+        g.sy = (long long)M3D_XBLOCK * rowp;
     } else if (all_fast) {
         g.sxa += M3D_XPAD; // x planes M3D_XPAD complex values apart beyond their rows
     }
@@ -263,18 +275,24 @@ void max_min(const std::vector<double>& a, double& mx, int& imx, double& mn)
 // --------------------------------------------------------------- engine
 // ------------------------------------------------------------------ autotuner
 // Variants of the FP32 y pass and x pass with the Green step: threads per
-// workgroup (256 / 512) and, for the x pass, the kz tile (xtile / 1.5x
-// wider).  Each is timed on the live buffers of the first FP32 solve of a
-// (grid, components, solves) shape; the choice is kept for the process.
+// workgroup (256 / 512), the radix plan of the line FFTs (radix 16 first or
+// radix 8 first) and, for the x pass, the kz tile (xtile / 1.5x wider).
+// Each is timed on the live buffers of the first FP32 solve of a (device,
+// grid, components, solves) shape.  The choices are kept for the process and
+// in a text file (M3D_TUNE_FILE, default $XDG_CACHE_HOME or
+// $HOME/.cache/mat3d_amd/tune.txt; "none" keeps them in the process only),
+// so later processes on the same device take them without timing.
 struct XyTune {
-    int xg_nt = 512, xg_wide = 0, y_nt = 512;
+    int xg_nt = 512, xg_wide = 0, xg_rp = 0, y_nt = 512, y_rp = 0;
+    int from_file = 0; // taken from the tuning file (1) or timed here (0)
 };
 
 struct TuneKey {
+    std::string dev; // device architecture and CU count
     int nx, ny, nz, c, ns;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(nx, ny, nz, c, ns) < std::tie(o.nx, o.ny, o.nz, o.c, o.ns);
+        return std::tie(dev, nx, ny, nz, c, ns) < std::tie(o.dev, o.nx, o.ny, o.nz, o.c, o.ns);
     }
 };
 
@@ -287,6 +305,84 @@ inline std::mutex& tune_mutex()
 {
     static std::mutex mu;
     return mu;
+}
+
+// entries of the tuning file are lines "dev nx ny nz c ns xg_nt xg_wide
+// xg_rp y_nt y_rp" (precision: FP32 passes)
+inline std::string tune_file_path()
+{
+    const char* e = std::getenv("M3D_TUNE_FILE");
+    if (e && *e) return (std::string(e) == "none") ? std::string() : std::string(e);
+    const char* x = std::getenv("XDG_CACHE_HOME");
+    const char* h = std::getenv("HOME");
+    if (x && *x) return std::string(x) + "/mat3d_amd/tune.txt";
+    if (h && *h) return std::string(h) + "/.cache/mat3d_amd/tune.txt";
+    return std::string();
+}
+
+// caller holds tune_mutex
+inline void tune_file_load()
+{
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const std::string path = tune_file_path();
+    if (path.empty()) return;
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char dev[128];
+    TuneKey k;
+    XyTune t;
+    while (std::fscanf(f, "%127s %d %d %d %d %d %d %d %d %d %d", dev, &k.nx, &k.ny, &k.nz, &k.c, &k.ns, &t.xg_nt,
+                       &t.xg_wide, &t.xg_rp, &t.y_nt, &t.y_rp) == 11) {
+        const bool ok = (t.xg_nt == 256 || t.xg_nt == 512) && (t.y_nt == 256 || t.y_nt == 512) &&
+                        (t.xg_wide == 0 || t.xg_wide == 1) && (t.xg_rp == 0 || t.xg_rp == 1) &&
+                        (t.y_rp == 0 || t.y_rp == 1);
+        if (!ok) continue; // a line this build does not know: timed again
+        k.dev = dev;
+        t.from_file = 1;
+        tune_cache()[k] = t;
+    }
+    std::fclose(f);
+}
+
+// caller holds tune_mutex; rewrites the file from the cache (tmp + rename)
+inline void tune_file_store()
+{
+    const std::string path = tune_file_path();
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        // create the directory chain (best effort: no file, no persistence)
+        std::string d;
+        for (size_t i = 0; i <= slash; ++i) {
+            if (path[i] == '/' && !d.empty()) (void)mkdir(d.c_str(), 0755);
+            d.push_back(path[i]);
+        }
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return;
+    for (const auto& kv : tune_cache()) {
+        const TuneKey& k = kv.first;
+        const XyTune& t = kv.second;
+        std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d\n", k.dev.c_str(), k.nx, k.ny, k.nz, k.c, k.ns, t.xg_nt,
+                     t.xg_wide, t.xg_rp, t.y_nt, t.y_rp);
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (ok) (void)std::rename(tmp.c_str(), path.c_str());
+    else (void)std::remove(tmp.c_str());
+}
+
+inline std::string device_tag()
+{
+    int dev = 0;
+    hipDeviceProp_t pr;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess) return "unknown";
+    std::string a = pr.gcnArchName;
+    const size_t c = a.find(':');
+    if (c != std::string::npos) a = a.substr(0, c);
+    return a + "x" + std::to_string(pr.multiProcessorCount);
 }
 
 template <typename PH>
@@ -554,7 +650,7 @@ struct Engine {
         return (unsigned)(check ? std::min<long long>(tiles, M3D_CHK_WG) : tiles);
     }
 
-    template <typename T, int KIND, int DIR, int WHICH, int L, int TK, int NTH>
+    template <typename T, int KIND, int DIR, int WHICH, int L, int TK, int NTH, int RP = 0>
     void xy_launch(const Geo& g, const Ptr<T>& p)
     {
         const int nkt = (g.Nzh + TK - 1) / TK;
@@ -564,10 +660,10 @@ struct Engine {
         constexpr bool LOOP = WHICH != BUF_MAIN;
         if (sm > 65536) {
             // gfx950: up to 160 KB of LDS per workgroup
-            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP>),
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP, RP>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
         }
-        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP>), grid, dim3(NTH), sm, st, g, p,
+        hipLaunchKernelGGL((fast::kxy<T, PH, KIND, DIR, WHICH, L, TK, NTH, LOOP, RP>), grid, dim3(NTH), sm, st, g, p,
                            prm<T>());
     }
 
@@ -584,12 +680,36 @@ struct Engine {
         if (TK != TKc) throw ArgError{M3D_ERR_UNSUPPORTED, "xy tile mismatch"};
         if constexpr (sizeof(T) == 4 && KIND == 1) {
             constexpr int TKw = fast::xtile_wide<T>(L, C);
+            constexpr bool RPS = fast::plans_differ(L);
             const XyTune& tu = tune32;
+            const int rp = RPS ? tu.xg_rp : 0;
             if (tu.xg_wide) {
-                if (tu.xg_nt == 512) xy_launch<T, KIND, DIR, WHICH, L, TKw, 512>(g, p);
-                else xy_launch<T, KIND, DIR, WHICH, L, TKw, 256>(g, p);
+                if (tu.xg_nt == 512) {
+                    if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKw, 512, RPS>(g, p);
+                    else xy_launch<T, KIND, DIR, WHICH, L, TKw, 512>(g, p);
+                } else {
+                    if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKw, 256, RPS>(g, p);
+                    else xy_launch<T, KIND, DIR, WHICH, L, TKw, 256>(g, p);
+                }
             } else {
-                if (tu.xg_nt == 512) xy_launch<T, KIND, DIR, WHICH, L, TKc, 512>(g, p);
+                if (tu.xg_nt == 512) {
+                    if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKc, 512, RPS>(g, p);
+                    else xy_launch<T, KIND, DIR, WHICH, L, TKc, 512>(g, p);
+                } else {
+                    if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKc, 256, RPS>(g, p);
+                    else xy_launch<T, KIND, DIR, WHICH, L, TKc, 256>(g, p);
+                }
+            }
+        } else if constexpr (sizeof(T) == 4 && KIND == 0 && WHICH == BUF_MAIN) {
+            // the main y passes (forward and inverse) take the tuned variant;
+            // the check's y pass keeps the default radix plan
+            constexpr bool RPS = fast::plans_differ(L);
+            const int rp = RPS ? tune32.y_rp : 0;
+            if (tune32.y_nt == 512) {
+                if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKc, 512, RPS>(g, p);
+                else xy_launch<T, KIND, DIR, WHICH, L, TKc, 512>(g, p);
+            } else {
+                if (RPS && rp) xy_launch<T, KIND, DIR, WHICH, L, TKc, 256, RPS>(g, p);
                 else xy_launch<T, KIND, DIR, WHICH, L, TKc, 256>(g, p);
             }
         } else if constexpr (sizeof(T) == 4 && KIND == 0) {
@@ -778,9 +898,10 @@ struct Engine {
     void tune_xy(Geo g)
     {
         g.nsolve = nsolve;
-        const TuneKey key{Nx, Ny, Nz, C, nsolve};
+        const TuneKey key{device_tag(), Nx, Ny, Nz, C, nsolve};
         {
             std::lock_guard<std::mutex> lk(tune_mutex());
+            tune_file_load();
             auto it = tune_cache().find(key);
             if (it != tune_cache().end()) {
                 tune32 = it->second;
@@ -800,37 +921,71 @@ struct Engine {
             return ms;
         };
         XyTune best = tune32;
+        best.from_file = 0;
         float bt = 1e30f;
         if (g.fx) {
-            for (int nt : {256, 512})
-                for (int wide : {0, 1}) {
-                    tune32.xg_nt = nt;
-                    tune32.xg_wide = wide;
-                    const float t = time_it([&]() { xgreen<float>(g, p); });
-                    if (t < bt) {
-                        bt = t;
-                        best.xg_nt = nt;
-                        best.xg_wide = wide;
+            for (int rp : {0, 1})
+                for (int nt : {256, 512})
+                    for (int wide : {0, 1}) {
+                        tune32.xg_nt = nt;
+                        tune32.xg_wide = wide;
+                        tune32.xg_rp = rp;
+                        const float t = time_it([&]() { xgreen<float>(g, p); });
+                        if (t < bt) {
+                            bt = t;
+                            best.xg_nt = nt;
+                            best.xg_wide = wide;
+                            best.xg_rp = rp;
+                        }
                     }
-                }
         }
         tune32 = best;
         bt = 1e30f;
         if (g.fy) {
-            for (int nt : {256, 512}) {
-                tune32.y_nt = nt;
-                const float t = time_it([&]() { ypass<float, -1, BUF_MAIN>(g, p); });
-                if (t < bt) {
-                    bt = t;
-                    best.y_nt = nt;
+            for (int rp : {0, 1})
+                for (int nt : {256, 512}) {
+                    tune32.y_nt = nt;
+                    tune32.y_rp = rp;
+                    const float t = time_it([&]() { ypass<float, -1, BUF_MAIN>(g, p); });
+                    if (t < bt) {
+                        bt = t;
+                        best.y_nt = nt;
+                        best.y_rp = rp;
+                    }
                 }
-            }
         }
         HCHK(hipGetLastError());
         tune32 = best;
         tuned32 = true;
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_cache()[key] = best;
+        tune_file_store();
+    }
+
+    // device buffers for n solves and, for an FP32 + polish engine, the
+    // autotuner's choice for this shape, ahead of solve(): the batch driver
+    // prepares every engine on one thread before the engines run together,
+    // so no allocation or timing overlaps another engine's kernels.  The
+    // timing runs the passes on the live (not yet loaded) buffers with every
+    // solve marked running.
+    void prepare(int n)
+    {
+        nsolve = n;
+        ensure_solves(n);
+        if (precision == M3D_FP32_POLISH && autotune && (!tuned32 || tuned_ns != n)) {
+            std::vector<Ctl> hctl(n);
+            for (Ctl& c : hctl) {
+                std::memset(&c, 0, sizeof c);
+                c.eq_it = -1;
+                c.cap = cap;
+            }
+            HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * n, hipMemcpyHostToDevice, st));
+            std::vector<int> rv(n, 0);
+            HCHK(hipMemcpyAsync(rve_of, rv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+            tune_xy(g32);
+            tuned_ns = n;
+            HCHK(hipStreamSynchronize(st));
+        }
     }
 
     // loads: nsolve x 6 (only the first C entries used); tol per phase
@@ -880,22 +1035,23 @@ struct Engine {
             for (int s = 0; s < nsolve; ++s) {
                 it32[s] = hctl[s].it;
                 info[s].iters_fp32 = hctl[s].it;
-                if (hctl[s].done == 1) {
-                    // borderline switch iterate: both criteria already within
-                    // rollback_margin x tol in FP32 -> the reference may stop
-                    // right here, so the last iterations are redone in FP64
-                    const bool border = hctl[s].dsum < rollback_margin * tol_f && hctl[s].deq < rollback_margin * tol_eq;
-                    jr[s] = border ? std::max(0, hctl[s].it - rollback) : hctl[s].it;
-                    hctl[s].it = jr[s];
-                    hctl[s].done = 0;
-                    hctl[s].need_cur = hctl[s].need_prev = 0;
-                    hctl[s].eq_it = -1;
-                    hctl[s].tol_f = tol_f;
-                    hctl[s].tol_eq = tol_eq;
-                } else {
-                    jr[s] = hctl[s].it;
-                    final_prec[s] = 0; // iteration cap reached in FP32: keep that state
-                }
+                // borderline switch iterate: both criteria already within
+                // rollback_margin x tol in FP32 -> the reference may stop
+                // right here, so the last iterations are redone in FP64.
+                // Iteration cap reached in FP32: the reference returns its
+                // FP64 iterate at the cap (FFT_analysis3D.cpp:785-792,
+                // FFT_thermal3D.cpp:474-480), so the last iterations are
+                // redone in FP64 too, up to the cap or to the exact stop rule
+                // if the FP32 phase only stalled on its floored tolerance
+                const bool capped = hctl[s].done != 1;
+                const bool border = hctl[s].dsum < rollback_margin * tol_f && hctl[s].deq < rollback_margin * tol_eq;
+                jr[s] = (capped || border) ? std::max(0, hctl[s].it - rollback) : hctl[s].it;
+                hctl[s].it = jr[s];
+                hctl[s].done = 0;
+                hctl[s].need_cur = hctl[s].need_prev = 0;
+                hctl[s].eq_it = -1;
+                hctl[s].tol_f = tol_f;
+                hctl[s].tol_eq = tol_eq;
             }
             HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, ptrs<float>(gf), ptrs<double>(gd), ctl);

@@ -42,10 +42,13 @@ constexpr int n_threes(int L) { return (L % 3 == 0) ? 1 + n_threes(L / 3) : 0; }
 #endif
 constexpr int n8(int e) { return e >= 3 ? ((e == 4) ? 0 : 1 + n8(e - 3)) : 0; }
 constexpr int rest2(int e) { return e >= 3 ? ((e == 4) ? 4 : rest2(e - 3)) : e; }
-// powers-of-two radices of 2^e, pass p (0 if p is past them)
-constexpr int pow2_radix(int e, int p)
+// powers-of-two radices of 2^e, pass p (0 if p is past them).  Radix plan
+// rp = 0 is the build's default (radix 16 first unless M3D_RADIX16 is 0),
+// rp = 1 the other one (radix 8 first, or 16 first when the default is 8);
+// the autotuner times both for the y pass and the x pass with the Green step.
+constexpr int pow2_radix(int e, int p, int rp = 0)
 {
-    if (!M3D_RADIX16) {
+    if ((M3D_RADIX16 != 0) == (rp != 0)) {
         const int a = n8(e), r = rest2(e);
         if (p < a) return 8;
         p -= a;
@@ -66,17 +69,25 @@ constexpr int pow2_radix(int e, int p)
     if (p == 0 && rem > 0) return 1 << rem;
     return 0;
 }
-constexpr int pow2_passes(int e)
+constexpr int pow2_passes(int e, int rp = 0)
 {
     int n = 0;
-    while (pow2_radix(e, n) != 0) ++n;
+    while (pow2_radix(e, n, rp) != 0) ++n;
     return n;
 }
-constexpr int npass(int L) { return pow2_passes(n_twos(L)) + n_threes(L); }
-constexpr int radix(int L, int p)
+constexpr int npass(int L, int rp = 0) { return pow2_passes(n_twos(L), rp) + n_threes(L); }
+constexpr int radix(int L, int p, int rp = 0)
 {
-    const int e = n_twos(L), a = pow2_passes(e);
-    return p < a ? pow2_radix(e, p) : 3;
+    const int e = n_twos(L), a = pow2_passes(e, rp);
+    return p < a ? pow2_radix(e, p, rp) : 3;
+}
+// the two radix plans of length L differ (else plan 1 is not worth timing)
+constexpr bool plans_differ(int L)
+{
+    if (npass(L, 0) != npass(L, 1)) return true;
+    for (int p = 0; p < npass(L, 0); ++p)
+        if (radix(L, p, 0) != radix(L, p, 1)) return true;
+    return false;
 }
 constexpr bool supported(int L) { return odd_part(L) == 1 || odd_part(L) == 3 || odd_part(L) == 9; }
 
@@ -260,13 +271,13 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
 }
 
 template <typename T, int L, int W, int NT, int DIR, int TWS, int P = 0, int Ns = 1, int WS = W, int G = W, int GS = 0,
-          int OFF = 0>
+          int OFF = 0, int RP = 0>
 __device__ __forceinline__ void fft(cx<T>* __restrict__ s, const cx<T>* __restrict__ tw)
 {
-    if constexpr (P < npass(L)) {
-        constexpr int R = radix(L, P);
+    if constexpr (P < npass(L, RP)) {
+        constexpr int R = radix(L, P, RP);
         pass<T, L, W, NT, R, Ns, DIR, TWS, WS, G, GS, OFF>(s, tw);
-        fft<T, L, W, NT, DIR, TWS, P + 1, Ns * R, WS, G, GS, OFF>(s, tw);
+        fft<T, L, W, NT, DIR, TWS, P + 1, Ns * R, WS, G, GS, OFF, RP>(s, tw);
     }
 }
 
@@ -741,14 +752,14 @@ struct has_green_hook<P, std::void_t<decltype(&P::template green<double>)>> : st
 // a tile already in LDS: forward FFT along the lines, then the Green step and
 // the inverse FFT (KIND 1) or the equilibrium partial sums (KIND 2; t is the
 // tile's index within its solve).  Returns with the tile in LDS (KIND 0, 1).
-template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NT>
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NT, int RP = 0>
 __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl& ctl, const T* __restrict__ prm, cx<T>* sl,
                                         const cx<T>* tw, double* red, int s, int o, int kz0, int nk, int t)
 {
     constexpr int C = PH::C;
     constexpr int W = TK * C;
     const int tid = lane_tid();
-    fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1>(sl, tw);
+    fft<T, L, W, NT, (KIND == 0 ? DIR : -1), 1, 0, 1, W, W, 0, 0, RP>(sl, tw);
 
     if constexpr (KIND == 1) {
         const int y = o;
@@ -766,7 +777,7 @@ __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl
             }
         }
         __syncthreads();
-        fft<T, L, W, NT, 1, 1>(sl, tw);
+        fft<T, L, W, NT, 1, 1, 0, 1, W, W, 0, 0, RP>(sl, tw);
     }
     if constexpr (KIND == 2) {
         const int y = o;
@@ -812,7 +823,7 @@ __device__ __forceinline__ bool xy_active(const Ctl& c)
 }
 
 // one workgroup per tile
-template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>()>
+template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>(), int RP = 0>
 __device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* __restrict__ prm_all, int b, cx<T>* sl,
                                         double* red)
 {
@@ -842,7 +853,7 @@ __device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* 
     // the Green step's reference medium (KIND 1 only: other users leave rve_of unset)
     const T* prm = nullptr;
     if constexpr (KIND == 1) prm = prm_all + 2 * p.rve_of[s];
-    xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT>(g, p, p.ctl[s], prm, sl, tw, red, s, o, kz0, nk, t);
+    xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT, RP>(g, p, p.ctl[s], prm, sl, tw, red, s, o, kz0, nk, t);
     if constexpr (KIND != 2) lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
 }
 
@@ -850,7 +861,7 @@ __device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* 
 // grid loops over the tiles of every solve and leaves at once when no solve
 // needs the check (see kz)
 template <typename T, typename PH, int KIND, int DIR, int WHICH, int L, int TK, int NTH = xy_threads<KIND, T>(),
-          bool LOOP = false>
+          bool LOOP = false, int RP = 0>
 __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T* __restrict__ prm_all)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -863,13 +874,14 @@ __global__ void __launch_bounds__(NTH) M3D_KXY_ATTR kxy(Geo g, Ptr<T> p, const T
         if (!__syncthreads_or(mine)) return;
         const int total = g.nsolve * (KIND == 0 ? g.Nx : g.Ny) * ((g.Nzh + TK - 1) / TK);
         for (int b = blockIdx.x; b < total; b += gridDim.x) {
-            xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH>(g, p, prm_all,
-                                                          xcd_split(b, total, total / max(1, g.nsolve)), sl, red);
+            xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH, RP>(g, p, prm_all,
+                                                              xcd_split(b, total, total / max(1, g.nsolve)), sl, red);
             __syncthreads();
         }
     } else {
-        xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH>(g, p, prm_all,
-                                                          xcd_split(blockIdx.x, gridDim.x, (int)gridDim.x / max(1, g.nsolve)), sl, red);
+        xy_body<T, PH, KIND, DIR, WHICH, L, TK, NTH, RP>(g, p, prm_all,
+                                                              xcd_split(blockIdx.x, gridDim.x, (int)gridDim.x / max(1, g.nsolve)), sl,
+                                                              red);
     }
 }
 

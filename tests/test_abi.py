@@ -89,3 +89,17 @@ def test_null_handles_of_the_other_rows():
     assert L.m3d_perc_create(None, 0, 8, 8, 2, 1, ct.byref(out)) == -1
     assert L.m3d_gen_balls(0, 8, 8, None, 0, 1, None) == -1
     assert L.m3d_gen_voronoi(8, 8, 8, None, 0, None) == -1
+
+
+def test_tuning_file_path_follows_the_environment(monkeypatch):
+    L = m.lib()
+    monkeypatch.setenv("M3D_TUNE_FILE", "/tmp/m3d_tune_probe.txt")
+    assert L.m3d_tuning_file() == b"/tmp/m3d_tune_probe.txt"
+    monkeypatch.setenv("M3D_TUNE_FILE", "none")
+    assert L.m3d_tuning_file() == b""
+    monkeypatch.delenv("M3D_TUNE_FILE")
+    monkeypatch.setenv("XDG_CACHE_HOME", "/tmp/xdg")
+    assert L.m3d_tuning_file() == b"/tmp/xdg/mat3d_amd/tune.txt"
+    # nothing timed or read in this process (no GPU): an empty report
+    rep = m.tuning_report()
+    assert rep["choices"] == []

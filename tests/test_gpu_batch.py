@@ -117,3 +117,98 @@ def test_batch_split_into_device_batches_gives_same_results(probs):
     for a, b in zip(full, split):
         assert np.array_equal(a["CHomo"], b["CHomo"])
         assert [x["iterations"] for x in a["info"]] == [x["iterations"] for x in b["info"]]
+
+
+# ------------------------------------------- batches on the specialised kernels
+def problems_fast():
+    """Voronoi polycrystals of the 64-RVE workload's kind (20-200 grains,
+    E in [50, 300] GPa, nu in [0.2, 0.35]) on grids whose line lengths have
+    the compile-time specialised kernels (32, 48 = 16 x 3, 64), one of them
+    made strain-based (its largest grain is the stiffest), so one launch
+    mixes bases, RVE coefficient tables and solves that stop at different
+    iterations."""
+    out = []
+    for i, (n, g) in enumerate([(32, 20), (48, 60), (32, 200), (48, 120), (64, 40), (32, 35)]):
+        ids = gen.voronoi(n, g, 1100 + i)
+        mat = grain_mat(g, 1200 + i)
+        if i == 5:
+            big = np.bincount(ids.ravel(), minlength=g).argmax()
+            mat[big, 0] = 350e9
+            mat[big, 1] = 0.2
+        out.append(((n, n, n), ids, mat))
+    return out
+
+
+@pytest.fixture(scope="module")
+def probs_fast():
+    return problems_fast()
+
+
+@pytest.fixture(scope="module")
+def oracle_fast(probs_fast):
+    out = []
+    for shape, ids, mat in probs_fast:
+        o = oracle.Elastic(*shape, "e")
+        o.material(mat, ids)
+        r = o.homogenization(1e-6, 1e-6)
+        r["strain_base"] = o.info()["strain_base"]
+        out.append(r)
+    return out
+
+
+def test_specialised_batch_fp64_matches_oracle(probs_fast, oracle_fast):
+    res, _ = run_batch(probs_fast, "fp64")
+    bases, iters = set(), set()
+    for i, (r, o) in enumerate(zip(res, oracle_fast)):
+        its = [x["iterations"] for x in r["info"]]
+        print(f"[fast batch fp64] rve {i} n {probs_fast[i][0][0]} grains {len(probs_fast[i][2])} "
+              f"base {'strain' if o['strain_base'] else 'stress'} iters {its} vs {o['iters']} "
+              f"relC {rel(r['CHomo'], o['C']):.2e}")
+        assert its == o["iters"]
+        assert rel(r["CHomo"], o["C"]) < 1e-9
+        assert rel(r["epHomo"], o["epHomo"]) < 1e-9
+        assert rel(r["sigHomo"], o["sigHomo"]) < 1e-9
+        bases.add(bool(o["strain_base"]))
+        iters.update(its)
+    assert bases == {False, True}
+    assert len(iters) > 1
+
+
+def test_specialised_batch_fp32_polish_matches_oracle(probs_fast, oracle_fast):
+    res, _ = run_batch(probs_fast, "fp32_polish")
+    for i, (r, o) in enumerate(zip(res, oracle_fast)):
+        e = rel(r["CHomo"], o["C"])
+        print(f"[fast batch fp32+polish] rve {i} relC {e:.2e} iters {[x['iterations'] for x in r['info']]} "
+              f"vs {o['iters']}")
+        assert e < 1e-6
+        assert all(x["status"] == 1 for x in r["info"])
+
+
+def test_full_voronoi_workload_uniform_moduli():
+    """The 64-RVE workload at its sizes ({32, 48, 64, 96, 128}^3, 20-200
+    grains, 384 solves) with every grain given the same moduli: the RVEs
+    are homogeneous, so C_eff equals the isotropic C exactly and every solve
+    stops after the reference's minimum of 2 iterations -- a property of the
+    whole batch path at full size, independent of the oracle."""
+    probs = gen.c3_batch(64, device=True)
+    E, nu = 120e9, 0.3
+    la, mu = nu * E / (1 + nu) / (1 - 2 * nu), E / (1 + nu) / 2
+    C = np.zeros((6, 6))
+    C[:3, :3] = la
+    C[np.arange(6), np.arange(6)] += np.r_[2 * mu * np.ones(3), 2 * mu * np.ones(3)]
+    for prec in ("fp64", "fp32_polish"):
+        b = m.Batch(prec)
+        for n, ids, mat in probs:
+            mat = mat.copy()
+            mat[:, 0], mat[:, 1] = E, nu
+            b.add(mat, ids)
+        b.homogenization(1e-6, 1e-6)
+        worst = 0.0
+        for i in range(len(probs)):
+            r = b.result(i)
+            worst = max(worst, rel(r["CHomo"], C))
+            assert [x["iterations"] for x in r["info"]] == [2] * 6, f"rve {i} {prec}"
+        print(f"[uniform voronoi batch {prec}] worst relC {worst:.2e}")
+        assert worst < (1e-12 if prec == "fp64" else 1e-6)
+        assert b.stats()["count"] == 64
+        b.close()

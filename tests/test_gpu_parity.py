@@ -330,3 +330,87 @@ def test_elastic_laminate_exact_256_fp32_polish():
         sg[w, :] = sgm
     Cref = np.linalg.inv(ep) @ sg if sb else np.linalg.inv(np.linalg.inv(sg) @ ep)
     assert rel(s.CHomo, Cref) < 1e-6
+
+
+def test_thermal_three_phase_laminate_512_material_n():
+    """The 512^3 conduction workload's phases through material_n: pores
+    k = 0.01 (phase 0, 10%), matrix k = 1 (phase 1, 75%), inclusions
+    k = 1000 (phase 2, 15%), here as layers normal to x.  The laminate model
+    (tests/laminate_model.py) predicts the base, the stop iteration of every
+    load case and the exact series/parallel K_eff; FP32 + polish must stop
+    where the FP64 rule does."""
+    import laminate_model as lm
+
+    n = 512
+    pat = np.ones(n, dtype=np.int32)
+    pat[:77] = 2
+    pat[200:251] = 0
+    L = lm.Layers(pat, "x", (n, n, n))
+    k = np.array([0.01, 1.0, 1000.0])
+    tol = 1e-7
+    model = lm.solve_thermal(L, k, tol, tol)
+    assert all(lm.margin(r, tol, tol) > 1e-3 for r in model["runs"])
+    s = m.FFT_analysis3d_thermal(n, n, n, precision="fp32_polish")
+    s.material_n(k, L.rve())
+    s.homogenization(tol, tol)
+    its = [i["iterations"] for i in s.info]
+    print("3-phase laminate 512", its, [r["iters"] for r in model["runs"]], s.KHomo.diagonal())
+    assert its == [r["iters"] for r in model["runs"]]
+    assert all(i["status"] == 1 for i in s.info)
+    f = L.frac
+    K = s.KHomo
+    # the model's own K at its stop iterate: <q>/<grad T> of each load case
+    # (the stop rule bounds the change per iteration, not the distance to
+    # the fixed point: at contrast 1e5 the in-plane iterate stops 1.6e-4 from
+    # the exact parallel value <k>)
+    for w in range(3):
+        r = model["runs"][w]
+        q = np.array([r["q"][p][w] for p in range(3)])
+        g = np.array([r["u"][p][w] if model["grad_base"] else r["q"][p][w] / k[p] for p in range(3)])
+        Km = float(f @ q) / float(f @ g)
+        assert abs(K[w, w] - Km) < 1e-6 * Km, (w, K[w, w], Km)
+    # the load along the normal (tensor index 2 is x) is exact after 2 iterations: the series value
+    ser = float(1 / (f @ (1 / k)))
+    assert abs(K[2, 2] - ser) < 1e-6 * ser
+
+
+@pytest.mark.parametrize("kind", ["elastic", "thermal"])
+def test_fp32_phase_reaching_the_cap_returns_the_fp64_iterate(kind):
+    """A solve whose FP32 phase reaches the iteration cap is finished in FP64:
+    its last iterations are redone as polish iterates, so the result is the
+    FP64 iterate at the cap that the reference returns
+    (FFT_analysis3D.cpp:785-792, FFT_thermal3D.cpp:474-480)."""
+    n = 32
+    if kind == "elastic":
+        ids = gen.fibres(n, 3, 0.3, seed=2, length=n // 2)
+        cap, tol = 12, 1e-6
+        o = oracle.Elastic(n, n, n, "e")
+        o.set_cap(cap)
+        o.material(gen.MAT_C2, ids)
+        r = o.homogenization(tol, tol)
+        ref, oi = r["C"], r["iters"]
+        s = m.FFT_Analysis3D(n, n, n, "e", precision="fp32_polish")
+        s.set_option("cap", cap)
+        s.material(gen.MAT_C2, ids)
+        s.homogenization(tol, tol)
+        got = s.CHomo
+    else:
+        ids = gen.multi_spheres(n, [(2, 0.15, 2, 4), (0, 0.10, 2, 4)], seed=5)
+        k = np.array([0.01, 1.0, 1000.0])
+        cap, tol = 40, 1e-7
+        o = oracle.Thermal(n, n, n)
+        o.set_cap(cap)
+        o.material_n(k, ids)
+        r = o.homogenization(tol, tol)
+        ref, oi = r["K"], r["iters"]
+        s = m.FFT_analysis3d_thermal(n, n, n, precision="fp32_polish")
+        s.set_option("cap", cap)
+        s.material_n(k, ids)
+        s.homogenization(tol, tol)
+        got = s.KHomo
+    its = [i["iterations"] for i in s.info]
+    print(kind, "capped", its, oi, [i["iters_fp64"] for i in s.info], rel(got, ref))
+    assert oi == [cap + 1] * len(oi)  # every load case of the oracle reaches the cap
+    assert its == oi
+    assert all(i["status"] == 2 and i["iters_fp64"] > 0 for i in s.info)
+    assert rel(got, ref) < 1e-6
