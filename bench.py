@@ -74,10 +74,34 @@ def iteration_bytes_per_voxel(Nz, C, r):
     return 3 * C * (r + 2 * r * (Nz // 2 + 1) / Nz) + 2 * C * r + 1
 
 
-def cpu_baseline(ids, mat, n, threads, iters=3):
+def job_roofline(bytes_per_s, basis):
+    """whole-job roofline: algorithmic bytes per voxel-iteration x voxel-iterations/s
+    against the HBM peak (no per-kernel traffic measured for this line)"""
+    gbs = bytes_per_s / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "basis": basis}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ids, mat, n, threads, iters=3, fft_threads=None):
+    """FP64 oracle iterations of load case 0.  fft_threads given: FFTs on that
+    many threads and the per-voxel loops on `threads` (the reference's own
+    setting is 3 FFTW threads and serial loops, FFT_analysis3D.cpp:101-102)."""
     import oracle
 
     o = oracle.Elastic(n, n, n, "e", nthreads=threads)
+    if fft_threads is not None:
+        o.set_threads(fft_threads, threads)
     o.material(mat, ids)
     o.run_fixed(0, 1)  # warm-up (page-in, plans)
     t0 = time.perf_counter()
@@ -186,12 +210,22 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # the box's CPU share for one GPU is 16 threads, whatever nproc shows
         threads = min(16, os.cpu_count() or 1)
         v, tcpu, done = cpu_baseline(ids, mat, n, threads)
+        print(f"[bench] cpu oracle: {v:.4e} voxel-it/s on {threads} threads", flush=True)
+        vr, tr, dr = cpu_baseline(ids, mat, n, 1, iters=1, fft_threads=min(3, threads))
+        print(f"[bench] cpu oracle, reference threading (3 FFT threads, serial voxel loops): {vr:.4e} voxel-it/s",
+              flush=True)
         cpu = {"value": v, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
                "sample": f"{n}^3 fibre RVE, load case 0, {done} accelerated FP64 iterations after 1 warm-up "
-                         f"({tcpu:.1f}s; oracle's own FP64 FFT, FFTW absent)"}
-        print(f"[bench] cpu oracle: {v:.4e} voxel-it/s on {threads} threads", flush=True)
+                         f"({tcpu:.1f}s; oracle's own FP64 FFT, FFTW absent; FFTs and voxel loops on all "
+                         f"{threads} threads)",
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "reference_threading": {"value": vr, "unit": "voxel-iters/s", "cores": min(3, threads),
+                                       "fft_threads": min(3, threads), "loop_threads": 1,
+                                       "sample": f"{n}^3 fibre RVE, load case 0, {dr} accelerated FP64 iterations "
+                                                 f"after 1 warm-up ({tr:.1f}s)"}}
 
     out = {
         "metric": METRIC,
@@ -221,7 +255,7 @@ def main():
         "roofline_iteration": {"bytes_per_voxel_iter": round(B, 2), "ms_per_iter_batch": round(it_ms, 4),
                                "achieved": round(it_gbs, 1), "frac": round(it_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
-        "autotune": tuning,
+        "autotune": dict(tuning, **m.tuning_report()),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -250,6 +284,9 @@ def _line(args, metric, value, unit, ms, dtype, workload, extra):
            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
            "data": "synthetic, seeded, generated in-process", "config": {"workload": workload}}
     out.update(extra)
+    import mat3d_amd as m
+
+    out.setdefault("autotune", m.tuning_report())
     print(json.dumps(out), flush=True)
 
 
@@ -277,11 +314,16 @@ def bench_voronoi_batch(args):
     dt = time.perf_counter() - t0
     st = b.stats()
     its = [x["iterations"] for i in range(len(probs)) for x in b.result(i)["info"]]
+    # algorithmic bytes of one homogenization of every RVE (FP32 accounting per grid)
+    byts = sum(n**3 * iteration_bytes_per_voxel(n, 6, 4) * sum(x["iterations"] for x in b.result(i)["info"])
+               for i, (n, _, _) in enumerate(probs))
     print(f"[bench] voronoi batch: {vi / dt:.4e} voxel-it/s, {dt / args.steps * 1e3:.1f} ms/step, "
           f"iterations min/mean/max {min(its)}/{np.mean(its):.1f}/{max(its)}, device ms {st['ms']:.1f}", flush=True)
     _line(args, "voxel-iters/s, 64-RVE Voronoi batch (384 solves)", vi / dt, "voxel-iters/s", dt / args.steps * 1e3,
           "f32+f64polish", "64 Voronoi RVEs {32..128}^3 x 6 load cases, accelerated scheme, tol 1e-6",
-          {"iterations_mean": float(np.mean(its)), "solves": len(its)})
+          {"iterations_mean": float(np.mean(its)), "solves": len(its),
+           "roofline": job_roofline(byts * args.steps / dt, "FP32 algorithmic bytes per voxel-iteration of each grid "
+                                                            "(194.1-197.5 B) x its voxel-iterations, per second")})
 
 
 def bench_thermal512(args):
@@ -307,9 +349,13 @@ def bench_thermal512(args):
     dt = time.perf_counter() - t0
     print(f"[bench] thermal512: {vi / dt:.4e} voxel-it/s, iterations {[d['iterations'] for d in s.info]} "
           f"status {[d['status'] for d in s.info]}, K diag {np.diag(s.KHomo)}", flush=True)
+    B = iteration_bytes_per_voxel(n, 3, 4)
     _line(args, "voxel-iters/s, 512^3 three-phase thermal", vi / dt, "voxel-iters/s", dt / args.steps * 1e3,
           "f32+f64polish", "512^3 thermal, 3 phases (k 0.01/1/1000), 3 load cases, tol 1e-7, cap 500",
-          {"iterations": [d["iterations"] for d in s.info], "K_eff_diag": list(np.diag(s.KHomo))})
+          {"iterations": [d["iterations"] for d in s.info], "status": [d["status"] for d in s.info],
+           "iters_fp64": [d["iters_fp64"] for d in s.info], "K_eff_diag": list(np.diag(s.KHomo)),
+           "roofline": job_roofline(B * vi / dt, f"FP32 algorithmic bytes per voxel-iteration ({B:.1f} B, C = 3) "
+                                                 "x voxel-iterations/s")})
 
 
 def bench_j2_192(args):
@@ -353,6 +399,11 @@ def bench_j2_192(args):
         extra["fp32_phase_opt_in"] = {"value": v32, "speedup_vs_fp64": v32 / v64, "iterations_fp32": i32["iters_fp32"],
                                       "iterations_fp64": i32["iters_fp64"], "prop_rel_diff": d_prop,
                                       "sigma_rel_diff": d_sig}
+    # FP64 algorithmic bytes per voxel-iteration with the J2 history read
+    # every iteration (sigma_old, eps_old: 48 B each, eps_p: 8 B)
+    B = iteration_bytes_per_voxel(n, 6, 8) + 104
+    extra["roofline"] = job_roofline(B * v64, f"FP64 algorithmic bytes per voxel-iteration with the J2 history "
+                                              f"({B:.1f} B) x voxel-iterations/s")
     _line(args, "voxel-iters/s, 192^3 J2 50 increments", v64, "voxel-iters/s", t64 * 1e3, "f64",
           "192^3 J2 matrix + elastic spheres, uniaxial 2% in 50 increments, basic scheme", extra)
 
