@@ -309,6 +309,19 @@ int m3d_gen_balls(int nx, int ny, int nz, const int32_t *balls, int nballs, int3
 int m3d_gen_cylinders(int nx, int ny, int nz, const double *cyl, int ncyl, double half, double radius, int32_t incl,
                       int batch, long long target, int32_t *ids, int *used, long long *count);
 
+/* No counterpart in the reference.  m3d_gen_cylinders for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight loops, harnesses
+ * that forbid hipMalloc): it calls no hipMalloc/hipFree.  ids_dev is the
+ * caller's DEVICE array of nx*ny*nz int32 (in/out, voxel i = z + nz*(y + ny*x));
+ * cyl stays a host array, staged into a fixed device buffer the library holds
+ * per GPU for the life of the process (8192 cylinders at most per call, more
+ * returns M3D_ERR_ARG; the per-workgroup counts live in the same buffer).
+ * Work runs on the null stream and the call returns when it is done.  Calls
+ * on one GPU share that buffer: calls that overlap on one GPU (other host
+ * threads) race. */
+int m3d_gen_cylinders_noalloc(int nx, int ny, int nz, const double *cyl, int ncyl, double half, double radius,
+                              int32_t incl, int batch, long long target, int32_t *ids_dev, int *used, long long *count);
+
 /* Periodic Voronoi cells: ids = index of the nearest of the g seeds (3
  * doubles each, inside the box) under the minimum-image distance, the first
  * seed on exact ties. */

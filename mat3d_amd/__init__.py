@@ -43,7 +43,7 @@ EXPORTS = [
     "m3d_batch_create", "m3d_batch_destroy", "m3d_batch_add", "m3d_batch_homogenization", "m3d_batch_results",
     "m3d_batch_stats", "m3d_batch_set_option", "m3d_batch_last_error",
     "m3d_rec_create", "m3d_rec_destroy", "m3d_rec_last_error", "m3d_rec_set_initial", "m3d_rec_run", "m3d_rec_result",
-    "m3d_gen_last_error", "m3d_gen_balls", "m3d_gen_cylinders", "m3d_gen_voronoi",
+    "m3d_gen_last_error", "m3d_gen_balls", "m3d_gen_cylinders", "m3d_gen_cylinders_noalloc", "m3d_gen_voronoi",
     "m3d_2d_create", "m3d_2d_destroy", "m3d_2d_last_error", "m3d_2d_material", "m3d_2d_set_load", "m3d_2d_run",
     "m3d_2d_homogenization", "m3d_2d_get_fields",
     "m3d_perc_create", "m3d_perc_destroy", "m3d_perc_last_error", "m3d_perc_run", "m3d_perc_result",
@@ -120,6 +120,8 @@ def lib():
             "m3d_gen_balls": (ip, [ip, ip, ip, i32p, ip, ct.c_int32, i32p]),
             "m3d_gen_cylinders": (ip, [ip, ip, ip, dp, ip, ct.c_double, ct.c_double, ct.c_int32, ip, ct.c_longlong,
                                        i32p, ipp, ct.POINTER(ct.c_longlong)]),
+            "m3d_gen_cylinders_noalloc": (ip, [ip, ip, ip, dp, ip, ct.c_double, ct.c_double, ct.c_int32, ip,
+                                               ct.c_longlong, vp, ipp, ct.POINTER(ct.c_longlong)]),
             "m3d_gen_voronoi": (ip, [ip, ip, ip, dp, ip, i32p]),
             "m3d_2d_create": (ip, [ip, ip, ct.POINTER(vp)]),
             "m3d_2d_destroy": (ip, [vp]),

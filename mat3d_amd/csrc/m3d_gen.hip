@@ -10,6 +10,7 @@
 
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -163,6 +164,32 @@ int run(const char* what, std::string& err, const std::function<void()>& f)
 
 thread_local std::string last_err;
 
+// Fixed device buffer of m3d_gen_cylinders_noalloc: the staged cylinder list
+// (12 doubles each) followed by the per-workgroup inclusion counts.  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never passes `scratch` itself to a kernel or a copy (on the host the
+// name is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr int NOALLOC_MAX_CYL = 8192;
+constexpr int NOALLOC_COUNT_BLOCKS = 1024;
+static __device__ __align__(256) double scratch[12 * NOALLOC_MAX_CYL + NOALLOC_COUNT_BLOCKS];
+
+double* scratch_ptr()
+{
+    static std::mutex mtx;
+    static std::vector<double*> per_device;
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mtx);
+    if ((int)per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev]) {
+        void* p = nullptr;
+        HCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        per_device[dev] = static_cast<double*>(p);
+    }
+    return per_device[dev];
+}
+
 } // namespace gen
 } // namespace m3d
 
@@ -243,5 +270,49 @@ int m3d_gen_voronoi(int nx, int ny, int nz, const double* pts, int g, int32_t* i
                            nx, ny, nz, (const double*)b.aux, g);
         HCHK(hipGetLastError());
         HCHK(hipMemcpy(ids, b.ids, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int m3d_gen_cylinders_noalloc(int nx, int ny, int nz, const double* cyl, int ncyl, double half, double radius,
+                              int32_t incl, int batch, long long target, int32_t* ids_dev, int* used, long long* count)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || ncyl < 0 || batch < 1 || !ids_dev || (ncyl > 0 && !cyl)) return M3D_ERR_ARG;
+    if (ncyl > gen::NOALLOC_MAX_CYL) {
+        gen::last_err = "gen_cylinders_noalloc: " + std::to_string(ncyl) + " cylinders exceed the fixed device buffer (" +
+                        std::to_string(gen::NOALLOC_MAX_CYL) + " at most per call)";
+        return M3D_ERR_ARG;
+    }
+    for (int k = 0; k < ncyl; ++k)
+        for (int a = 0; a < 3; ++a)
+            if (cyl[12 * (size_t)k + 9 + a] < cyl[12 * (size_t)k + 6 + a]) return M3D_ERR_ARG;
+    return gen::run("gen_cylinders_noalloc", gen::last_err, [&]() {
+        const long long n = (long long)nx * ny * nz;
+        const int nb = gen::NOALLOC_COUNT_BLOCKS;
+        double* sp = gen::scratch_ptr();
+        double* cyl_d = sp;
+        unsigned long long* cnt = reinterpret_cast<unsigned long long*>(sp + 12 * (size_t)gen::NOALLOC_MAX_CYL);
+        if (ncyl) HCHK(hipMemcpy(cyl_d, cyl, sizeof(double) * 12 * (size_t)ncyl, hipMemcpyHostToDevice));
+        unsigned long long part[gen::NOALLOC_COUNT_BLOCKS];
+        // every count overwrites all nb entries, so nothing of an earlier
+        // call's contents is read
+        auto counted = [&]() {
+            hipLaunchKernelGGL(gen::k_count, dim3(nb), dim3(gen::NT), 0, nullptr, ids_dev, n, incl, cnt);
+            HCHK(hipGetLastError());
+            HCHK(hipMemcpy(part, cnt, sizeof(part), hipMemcpyDeviceToHost));
+            long long s = 0;
+            for (unsigned long long v : part) s += (long long)v;
+            return s;
+        };
+        long long c = counted();
+        int k = 0;
+        while (c < target && k + batch <= ncyl) {
+            hipLaunchKernelGGL(gen::k_cylinders, dim3(batch), dim3(gen::NT), 0, nullptr, ids_dev, nx, ny, nz,
+                               (const double*)cyl_d + 12 * (size_t)k, half, radius * radius, incl);
+            k += batch;
+            c = counted();
+        }
+        HCHK(hipGetLastError());
+        if (used) *used = k;
+        if (count) *count = c;
     });
 }
