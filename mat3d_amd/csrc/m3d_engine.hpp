@@ -910,10 +910,13 @@ struct Engine {
             }
         }
         Ptr<float> p = ptrs<float>(g);
+        // each variant's time is the best of `rounds` timings of `reps`
+        // launches, the variants interleaved within every round, so a drift
+        // of the clocks during the tuning does not decide the choice
+        constexpr int rounds = 4, reps = 2;
         auto time_it = [&](auto&& launch) {
-            launch();
             HCHK(hipEventRecord(ev0, st));
-            for (int r = 0; r < 3; ++r) launch();
+            for (int r = 0; r < reps; ++r) launch();
             HCHK(hipEventRecord(ev1, st));
             HCHK(hipEventSynchronize(ev1));
             float ms = 0;
@@ -922,37 +925,41 @@ struct Engine {
         };
         XyTune best = tune32;
         best.from_file = 0;
-        float bt = 1e30f;
         if (g.fx) {
+            std::vector<std::array<int, 3>> vars; // {threads, wide, radix plan}
             for (int rp : {0, 1})
                 for (int nt : {256, 512})
-                    for (int wide : {0, 1}) {
-                        tune32.xg_nt = nt;
-                        tune32.xg_wide = wide;
-                        tune32.xg_rp = rp;
-                        const float t = time_it([&]() { xgreen<float>(g, p); });
-                        if (t < bt) {
-                            bt = t;
-                            best.xg_nt = nt;
-                            best.xg_wide = wide;
-                            best.xg_rp = rp;
-                        }
-                    }
+                    for (int wide : {0, 1}) vars.push_back({nt, wide, rp});
+            std::vector<float> bt(vars.size(), 1e30f);
+            for (int r = 0; r <= rounds; ++r)
+                for (size_t v = 0; v < vars.size(); ++v) {
+                    tune32.xg_nt = vars[v][0];
+                    tune32.xg_wide = vars[v][1];
+                    tune32.xg_rp = vars[v][2];
+                    const float t = time_it([&]() { xgreen<float>(g, p); });
+                    if (r > 0) bt[v] = std::min(bt[v], t); // round 0 warms every variant up
+                }
+            const size_t k = std::min_element(bt.begin(), bt.end()) - bt.begin();
+            best.xg_nt = vars[k][0];
+            best.xg_wide = vars[k][1];
+            best.xg_rp = vars[k][2];
         }
         tune32 = best;
-        bt = 1e30f;
         if (g.fy) {
+            std::vector<std::array<int, 2>> vars; // {threads, radix plan}
             for (int rp : {0, 1})
-                for (int nt : {256, 512}) {
-                    tune32.y_nt = nt;
-                    tune32.y_rp = rp;
+                for (int nt : {256, 512}) vars.push_back({nt, rp});
+            std::vector<float> bt(vars.size(), 1e30f);
+            for (int r = 0; r <= rounds; ++r)
+                for (size_t v = 0; v < vars.size(); ++v) {
+                    tune32.y_nt = vars[v][0];
+                    tune32.y_rp = vars[v][1];
                     const float t = time_it([&]() { ypass<float, -1, BUF_MAIN>(g, p); });
-                    if (t < bt) {
-                        bt = t;
-                        best.y_nt = nt;
-                        best.y_rp = rp;
-                    }
+                    if (r > 0) bt[v] = std::min(bt[v], t);
                 }
+            const size_t k = std::min_element(bt.begin(), bt.end()) - bt.begin();
+            best.y_nt = vars[k][0];
+            best.y_rp = vars[k][1];
         }
         HCHK(hipGetLastError());
         tune32 = best;
