@@ -44,6 +44,14 @@ constexpr size_t LDS_MAX = 65536;
 #define M3D_XPAD 128
 #endif
 
+// byte alignment of the spectrum rows of the specialised kernels (0: rows
+// packed at Nzh*C complex values).  64-byte rows were measured slower in an
+// A/B on one box at 256^3 (z pass +6%, y passes +7%, x pass with the Green
+// step -1%): the default keeps the rows packed.
+#ifndef M3D_ROW_ALIGN
+#define M3D_ROW_ALIGN 0
+#endif
+
 // workgroups of a check-pass launch (the check kernels loop over their tiles)
 #ifndef M3D_CHK_WG
 #define M3D_CHK_WG 32768
@@ -199,8 +207,9 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
     // as partial sectors (FP32, C = 6, 256^3: rows of 129 slots, 6192 bytes,
     // padded to 132 slots)
     long long rowp = rowc;
+    constexpr long long row_align = M3D_ROW_ALIGN > 0 ? M3D_ROW_ALIGN : 1;
     if (all_fast)
-        while ((rowp * (long long)csz) % 64) rowp += C;
+        while ((rowp * (long long)csz) % row_align) rowp += C;
     g.lxb = 0;
     g.xbm = 0;
     g.sxb = rowp;

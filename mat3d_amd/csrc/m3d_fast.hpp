@@ -218,6 +218,41 @@ __device__ __forceinline__ int lane_tid()
     return t;
 }
 
+// Index arithmetic at full rate: on CDNA the 32-bit integer multiply
+// (v_mul_lo_u32, v_mul_hi_u32) and the 64-bit multiply-add issue at a quarter
+// of the rate of v_mul_u32_u24, and the x pass with the Green step is bound by
+// VALU issue.  cdiv<D, XMAX>(x) is x / D for 0 <= x < XMAX: a shift for powers
+// of two, else one 24-bit multiply by M = ceil(2^S / D) and a shift, with S the
+// smallest shift for which x * M fits 32 bits and XMAX * (M D - 2^S) <= 2^S
+// (then floor(x M / 2^S) = floor(x / D) for every x < XMAX).
+template <int D, int XMAX>
+struct FastDiv {
+    static constexpr int pick()
+    {
+        for (int S = 0; S < 32; ++S) {
+            const unsigned long long M = ((1ull << S) + D - 1) / D;
+            const unsigned long long e = M * D - (1ull << S);
+            if (M < (1ull << 24) && (unsigned long long)XMAX * M < (1ull << 32) && (unsigned long long)XMAX * e <= (1ull << S))
+                return S;
+        }
+        return -1;
+    }
+    static constexpr int S = pick();
+    static constexpr unsigned M = S < 0 ? 0u : (unsigned)(((1ull << (S < 0 ? 0 : S)) + D - 1) / D);
+};
+
+template <int D, int XMAX>
+__device__ __forceinline__ int cdiv(int x)
+{
+    static_assert(D > 0 && XMAX > 0 && XMAX <= (1 << 24), "cdiv range");
+    if constexpr ((D & (D - 1)) == 0) {
+        return (int)((unsigned)x / (unsigned)D);
+    } else {
+        static_assert(FastDiv<D, XMAX>::S >= 0, "no 24-bit reciprocal for this divisor and range");
+        return (int)(__umul24((unsigned)x, FastDiv<D, XMAX>::M) >> FastDiv<D, XMAX>::S);
+    }
+}
+
 // --------------------------------------------------------------- passes
 template <int DIR, typename T>
 __device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w)
@@ -244,8 +279,13 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
     for (int i = 0; i < IPT; ++i) {
         const int it = tid + i * NT;
         if (items % NT == 0 || it < items) {
-            const int l0 = it % W, j = it / W, k = j % Ns;
-            const int l = (l0 / G) * GS + OFF + l0 % G;
+            const int j = cdiv<W, IPT * NT>(it), l0 = it - j * W, k = j % Ns;
+            int l;
+            if constexpr (G == W) l = OFF + l0;
+            else {
+                const int gq = cdiv<G, W>(l0);
+                l = gq * GS + OFF + (l0 - gq * G);
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 cx<T> a = s[(j + r * nR) * WS + l];
@@ -260,8 +300,13 @@ __device__ __forceinline__ void pass(cx<T>* __restrict__ s, const cx<T>* __restr
     for (int i = 0; i < IPT; ++i) {
         const int it = tid + i * NT;
         if (items % NT == 0 || it < items) {
-            const int l0 = it % W, j = it / W, k = j % Ns;
-            const int l = (l0 / G) * GS + OFF + l0 % G;
+            const int j = cdiv<W, IPT * NT>(it), l0 = it - j * W, k = j % Ns;
+            int l;
+            if constexpr (G == W) l = OFF + l0;
+            else {
+                const int gq = cdiv<G, W>(l0);
+                l = gq * GS + OFF + (l0 - gq * G);
+            }
             const int ob = (j / Ns) * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) s[(ob + r * Ns) * WS + l] = v[i][r];
@@ -344,6 +389,15 @@ struct RowMap {
     {
         return (long long)(r >> sh) * a + (long long)(r & mask) * b;
     }
+    // offset of row r < ROWS: plain rows (one stride) whose offsets stay
+    // below 2^32 take one full-rate 24-bit multiply (a uniform branch)
+    template <int ROWS>
+    __device__ __forceinline__ long long off(int r) const
+    {
+        if (sh == 0 && mask == 0 && a < (long long)(0xffffffffull / ROWS) && a < (1ll << 24))
+            return (long long)__umul24((unsigned)r, (unsigned)a);
+        return (*this)(r);
+    }
 };
 __device__ __forceinline__ RowMap uniform_rows(long long stride) { return RowMap{stride, 0, 0, 0}; }
 
@@ -368,8 +422,8 @@ __device__ __forceinline__ void rows_load(V (&r)[PER], const cx<T>* __restrict__
     for (int i = 0; i < RT::PER; ++i) {
         const int e = t0 + i * NT;
         if (RT::N % NT == 0 || e < RT::N) {
-            const int row = e / RT::WG, j = (e - row * RT::WG) * RT::G;
-            r[i] = j < wv ? *reinterpret_cast<const V*>(base + rm(row) + j) : V(0);
+            const int row = cdiv<RT::WG, RT::PER * NT>(e), j = (e - row * RT::WG) * RT::G;
+            r[i] = j < wv ? *reinterpret_cast<const V*>(base + rm.template off<ROWS>(row) + j) : V(0);
         }
     }
 }
@@ -384,7 +438,7 @@ __device__ __forceinline__ void rows_commit(cx<T>* __restrict__ sl, const V (&r)
     for (int i = 0; i < RT::PER; ++i) {
         const int e = t0 + i * NT;
         if (RT::N % NT == 0 || e < RT::N) {
-            const int row = e / RT::WG, j = (e - row * RT::WG) * RT::G;
+            const int row = cdiv<RT::WG, RT::PER * NT>(e), j = (e - row * RT::WG) * RT::G;
             *reinterpret_cast<V*>(sl + row * W + j) = r[i];
         }
     }
@@ -411,8 +465,9 @@ __device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>*
     for (int i = 0; i < PER; ++i) {
         const int e = t0 + i * NT;
         if (N % NT == 0 || e < N) {
-            const int row = e / WG, j = (e - row * WG) * G;
-            if (j < wv) *reinterpret_cast<V*>(base + rm(row) + j) = *reinterpret_cast<const V*>(sl + row * W + j);
+            const int row = cdiv<WG, PER * NT>(e), j = (e - row * WG) * G;
+            if (j < wv)
+                *reinterpret_cast<V*>(base + rm.template off<ROWS>(row) + j) = *reinterpret_cast<const V*>(sl + row * W + j);
         }
     }
 }
@@ -731,13 +786,13 @@ template <typename T, int C, int KIND, int WHICH>
 __device__ __forceinline__ const cx<T>* xy_rows(const Geo& g, const Ptr<T>& p, int s, int o, int kz0, RowMap& rm)
 {
     if (KIND == 0) {
-        // tile: plane x = o, lines along y
-        rm = RowMap{0, g.sy, 30, (1 << 30) - 1};
+        // tile: plane x = o, lines along y (row y at y * sy)
+        rm = RowMap{g.sy, 0, 0, 0};
         return (WHICH == BUF_MAIN ? p.S : p.K) + (long long)s * p.sS + (long long)(o >> g.lxb) * g.sxa +
                (long long)(o & g.xbm) * g.sxb + (long long)kz0 * C;
     }
     // tile: row y = o, lines along x; row (x, y) at (x >> lxb) * sxa + (x & xbm) * sxb + y * sy
-    rm = RowMap{g.sxa, g.sxb, g.lxb, g.xbm};
+    rm = (g.lxb == 0 && g.xbm == 0) ? RowMap{g.sxa, 0, 0, 0} : RowMap{g.sxa, g.sxb, g.lxb, g.xbm};
     return (KIND == 1 ? p.S : p.K) + (long long)s * p.sS + (long long)o * g.sy + (long long)kz0 * C;
 }
 
@@ -763,8 +818,9 @@ __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl
 
     if constexpr (KIND == 1) {
         const int y = o;
-        for (int f = tid; f < L * nk; f += NT) {
-            const int kx = f / nk, kzl = f - kx * nk;
+        GreenK<T> K{};
+        if constexpr (!has_green_hook<PH>::value) K = green_k_of<T, C>(prm);
+        auto step = [&](int kx, int kzl) {
             const int kz = kz0 + kzl;
             cx<T>* v = sl + kx * W + kzl * C;
             if constexpr (has_green_hook<PH>::value) {
@@ -773,7 +829,19 @@ __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl
 #pragma unroll
                 for (int c = 0; c < C; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
             } else {
-                green_apply<T, PH>(g, ctl.sb, prm, kx, y, kz, v);
+                green_apply_k<T, PH>(g, ctl.sb, K, kx, y, kz, v);
+            }
+        };
+        // full kz tiles (all but the last) divide by the constant TK
+        if (nk == TK) {
+            for (int f = tid; f < L * TK; f += NT) {
+                const int kx = cdiv<TK, L * TK + NT>(f);
+                step(kx, f - kx * TK);
+            }
+        } else {
+            for (int f = tid; f < L * nk; f += NT) {
+                const int kx = f / nk;
+                step(kx, f - kx * nk);
             }
         }
         __syncthreads();
@@ -783,7 +851,7 @@ __device__ __forceinline__ void xy_tile(const Geo& g, const Ptr<T>& p, const Ctl
         const int y = o;
         double acc = 0.0;
         for (int f = tid; f < L * nk; f += NT) {
-            const int kx = f / nk, kzl = f - kx * nk;
+            const int kx = nk == TK ? cdiv<TK, L * TK + NT>(f) : f / nk, kzl = f - kx * nk;
             const int kz = kz0 + kzl;
             const cx<T>* v = sl + kx * W + kzl * C;
             double a2[C];

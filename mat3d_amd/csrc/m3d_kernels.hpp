@@ -178,6 +178,43 @@ struct Thermal {
 };
 
 // ------------------------------------------------------- Green operators
+// per-solve constants of the isotropic Green operators, computed once per
+// tile instead of per frequency (same values and operation order as the
+// per-frequency expressions they replace)
+template <typename T>
+struct GreenK {
+    T la0, mu0;
+    T d2;   // (la0 + mu0) / (mu0 (la0 + 2 mu0))
+    T c3;   // 2 la0 mu0 / (la0 + 2 mu0)
+    T m2;   // 2 mu0
+    T h2mu; // 1 / (2 mu0): FP32 multiplies by it where FP64 divides
+    T k0;   // thermal reference conductivity
+};
+
+template <typename T>
+__device__ __forceinline__ GreenK<T> green_k(T la0, T mu0)
+{
+    GreenK<T> K;
+    K.la0 = la0;
+    K.mu0 = mu0;
+    K.d2 = (la0 + mu0) / (mu0 * (la0 + T(2) * mu0));
+    K.c3 = T(2) * la0 * mu0 / (la0 + T(2) * mu0);
+    K.m2 = T(2) * mu0;
+    K.h2mu = T(1) / (T(2) * mu0);
+    K.k0 = T(0);
+    return K;
+}
+
+// the constants of a solve's medium: elastic prm = {la0, mu0}, thermal {k0}
+template <typename T, int C>
+__device__ __forceinline__ GreenK<T> green_k_of(const T* prm)
+{
+    if constexpr (C == 6) return green_k<T>(prm[0], prm[1]);
+    GreenK<T> K{};
+    K.k0 = prm[0];
+    return K;
+}
+
 // Strain base, Gamma0 (FFT_analysis3D.cpp:223-224) contracted with a
 // symmetric tau (all shear pairs summed): closed form
 //   (G:t)_ij = (xi_i (t xi)_j + xi_j (t xi)_i) / (2 mu0 |xi|^2)
@@ -186,11 +223,15 @@ struct Thermal {
 //   (D:t)_ij = c3 P_ij tr(P t) + 2 mu0 (P t P)_ij,  c3 = 2 la0 mu0/(la0+2mu0)
 // Returns G:t (complex, per Voigt component).
 template <typename T>
-__device__ __forceinline__ void green_elastic(const T xi[3], const cx<T> t[6], cx<T> o[6], int strain_base,
-                                              T la0, T mu0)
+__device__ __forceinline__ void green_elastic_k(const T xi[3], const cx<T> t[6], cx<T> o[6], int strain_base,
+                                                const GreenK<T>& K)
 {
     const T n2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-    const T inv = T(1) / n2;
+    // FP32: the hardware reciprocal (1 ulp) in place of the IEEE division
+    // sequence; FP32 iterates are refined in FP64, whose path divides exactly
+    T inv;
+    if constexpr (sizeof(T) == 4) inv = __builtin_amdgcn_rcpf(n2);
+    else inv = T(1) / n2;
     // w = t xi, tensor from Voigt: 0:(0,0) 1:(1,1) 2:(2,2) 3:(1,2) 4:(0,2) 5:(0,1)
     cx<T> w0 = cscale(t[0], xi[0]) + cscale(t[5], xi[1]) + cscale(t[4], xi[2]);
     cx<T> w1 = cscale(t[5], xi[0]) + cscale(t[1], xi[1]) + cscale(t[3], xi[2]);
@@ -200,9 +241,10 @@ __device__ __forceinline__ void green_elastic(const T xi[3], const cx<T> t[6], c
     const int pi[6] = {0, 1, 2, 1, 0, 0};
     const int pj[6] = {0, 1, 2, 2, 2, 1};
     if (strain_base) {
-        const T a = inv / (T(2) * mu0);
-        const T d2 = (la0 + mu0) / (mu0 * (la0 + T(2) * mu0));
-        const T b = d2 * inv * inv;
+        T a;
+        if constexpr (sizeof(T) == 4) a = inv * K.h2mu;
+        else a = inv / (T(2) * K.mu0);
+        const T b = K.d2 * inv * inv;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int i = pi[q], j = pj[q];
@@ -210,8 +252,6 @@ __device__ __forceinline__ void green_elastic(const T xi[3], const cx<T> t[6], c
             o[q] = r - cscale(s, b * xi[i] * xi[j]);
         }
     } else {
-        const T c3 = T(2) * la0 * mu0 / (la0 + T(2) * mu0);
-        const T m2 = T(2) * mu0;
         // n n = xi xi / n2 ; s/n2 = n.t.n ; w/|xi| = t n
         cx<T> trt = t[0] + t[1] + t[2];
         cx<T> sn = cscale(s, inv);                // n.t.n
@@ -223,24 +263,35 @@ __device__ __forceinline__ void green_elastic(const T xi[3], const cx<T> t[6], c
             const T Pij = (i == j ? T(1) : T(0)) - nn;
             // (P t P)_ij = t_ij - n_i w'_j - w'_i n_j + sn n_i n_j, w' = t n
             cx<T> ptp = t[q] - cscale(cscale(w[j], xi[i]) + cscale(w[i], xi[j]), inv) + cscale(sn, nn);
-            o[q] = cscale(trPt, c3 * Pij) + cscale(ptp, m2);
+            o[q] = cscale(trPt, K.c3 * Pij) + cscale(ptp, K.m2);
         }
     }
 }
 
-// Thermal gradient base: G_ij = xi_i xi_j / (xi.K0.xi)   (FFT_thermal3D.cpp:126-136)
+template <typename T>
+__device__ __forceinline__ void green_elastic(const T xi[3], const cx<T> t[6], cx<T> o[6], int strain_base,
+                                              T la0, T mu0)
+{
+    green_elastic_k<T>(xi, t, o, strain_base, green_k<T>(la0, mu0));
+}
+
 // Thermal flux base:     D_ij = k0 (delta_ij - xi_i xi_j/|xi|^2) (:161-169)
 template <typename T>
 __device__ __forceinline__ void green_thermal(const T xi[3], const cx<T> t[3], cx<T> o[3], int grad_base, T k0)
 {
     const T n2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
     cx<T> s = cscale(t[0], xi[0]) + cscale(t[1], xi[1]) + cscale(t[2], xi[2]);
+    // FP32: hardware reciprocal, as in green_elastic
+    auto rcp = [](T x) {
+        if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+        else return T(1) / x;
+    };
     if (grad_base) {
-        const T a = T(1) / (k0 * n2);
+        const T a = rcp(k0 * n2);
 #pragma unroll
         for (int c = 0; c < 3; ++c) o[c] = cscale(s, a * xi[c]);
     } else {
-        const T a = T(1) / n2;
+        const T a = rcp(n2);
 #pragma unroll
         for (int c = 0; c < 3; ++c) o[c] = cscale(t[c] - cscale(s, a * xi[c]), k0);
     }
@@ -571,7 +622,7 @@ __device__ __forceinline__ void freq_th(const Geo& g, int sb, int kx, int ky, in
 }
 
 template <typename T, typename PH>
-__device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, int kx, int ky, int kz, cx<T>* v)
+__device__ __forceinline__ void green_apply_k(const Geo& g, int sb, const GreenK<T>& K, int kx, int ky, int kz, cx<T>* v)
 {
     constexpr int C = PH::C;
     int f[3];
@@ -581,8 +632,8 @@ __device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, 
     cx<T> in[C], o[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) in[c] = v[c];
-    if constexpr (C == 6) green_elastic<T>(xi, in, o, sb, prm[0], prm[1]);
-    else green_thermal<T>(xi, in, o, sb, prm[0]);
+    if constexpr (C == 6) green_elastic_k<T>(xi, in, o, sb, K);
+    else green_thermal<T>(xi, in, o, sb, K.k0);
     // c2r of a multi-dimensional half spectrum keeps only the Hermitian part
     // of the kz=0 and kz=Nz/2 planes: average with the operator of the
     // conjugate-partner index when its frequency vector is not -xi.
@@ -595,8 +646,8 @@ __device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, 
         if (fp[0] != -f[0] || fp[1] != -f[1] || fp[2] != -f[2]) {
             T xp[3] = {T(fp[0]), T(fp[1]), T(fp[2])};
             cx<T> o2[C];
-            if constexpr (C == 6) green_elastic<T>(xp, in, o2, sb, prm[0], prm[1]);
-            else green_thermal<T>(xp, in, o2, sb, prm[0]);
+            if constexpr (C == 6) green_elastic_k<T>(xp, in, o2, sb, K);
+            else green_thermal<T>(xp, in, o2, sb, K.k0);
 #pragma unroll
             for (int c = 0; c < C; ++c) o[c] = cscale(o[c] + o2[c], T(0.5));
         }
@@ -604,6 +655,12 @@ __device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, 
     const T sc = T(-g.invN);
 #pragma unroll
     for (int c = 0; c < C; ++c) v[c] = cscale(o[c], sc);
+}
+
+template <typename T, typename PH>
+__device__ __forceinline__ void green_apply(const Geo& g, int sb, const T* prm, int kx, int ky, int kz, cx<T>* v)
+{
+    green_apply_k<T, PH>(g, sb, green_k_of<T, PH::C>(prm), kx, ky, kz, v);
 }
 
 // prm: elastic {la0, mu0}; thermal {k0}
