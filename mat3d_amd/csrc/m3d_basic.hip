@@ -35,7 +35,7 @@ struct Comp6 {
 // reads S12): an inverse y pass, then here the c2r and the constitutive
 // update (the history does not change within an increment).  The damage
 // variable is written every iteration: the next one compares with it.
-template <typename T, int BEH, int MODE, int LZ, int TY>
+template <typename T, int BEH, int MODE, int LZ, int TY, bool LIN = false>
 __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 {
     constexpr bool FIN = MODE == BMODE_FINAL;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
                 T so[6], eo[6], phi, dep, seff;
                 fast::vload<T, 6>(p.sig_old + vg * 6, so);
                 fast::vload<T, 6>(p.eps_old + vg * 6, eo);
-                if (!j2_update<T>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+                if (!j2_update<T, LIN>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
                 if (STORE) {
                     p.phi[vg] = phi;
                     p.dep[vg] = dep;
@@ -309,24 +309,24 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
 // ---------------------------------------------------------------- launchers
 namespace {
 
-template <typename T, int BEH, int MODE, int LZ>
+template <typename T, int BEH, int MODE, int LZ, bool LIN>
 bool z_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (g.Nz != 2 * LZ) return false;
     constexpr int TY = zlines(LZ, 2 * LZ);
     if (g.TY != TY) throw std::runtime_error("basic z tile mismatch");
     const size_t sm = ((size_t)(LZ + 1) * TY * 12 + 2 * LZ) * sizeof(cx<T>);
-    hipLaunchKernelGGL((kzb<T, BEH, MODE, LZ, TY>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
+    hipLaunchKernelGGL((kzb<T, BEH, MODE, LZ, TY, LIN>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
     return true;
 }
 
-template <typename T, int BEH, int MODE>
+template <typename T, int BEH, int MODE, bool LIN = false>
 void z_mode(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
-    const bool ok = z_at<T, BEH, MODE, 128>(g, p, st) || z_at<T, BEH, MODE, 96>(g, p, st) ||
-                    z_at<T, BEH, MODE, 64>(g, p, st) || z_at<T, BEH, MODE, 48>(g, p, st) ||
-                    z_at<T, BEH, MODE, 32>(g, p, st) || z_at<T, BEH, MODE, 24>(g, p, st) ||
-                    z_at<T, BEH, MODE, 16>(g, p, st);
+    const bool ok = z_at<T, BEH, MODE, 128, LIN>(g, p, st) || z_at<T, BEH, MODE, 96, LIN>(g, p, st) ||
+                    z_at<T, BEH, MODE, 64, LIN>(g, p, st) || z_at<T, BEH, MODE, 48, LIN>(g, p, st) ||
+                    z_at<T, BEH, MODE, 32, LIN>(g, p, st) || z_at<T, BEH, MODE, 24, LIN>(g, p, st) ||
+                    z_at<T, BEH, MODE, 16, LIN>(g, p, st);
     if (!ok) throw std::runtime_error("no basic z kernel for this grid");
 }
 
@@ -369,9 +369,14 @@ bool x_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
 } // namespace
 
 template <typename T>
-void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st)
+void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st, bool lin)
 {
-    if (beh == BEH_EP) {
+    if (beh == BEH_EP && lin && sizeof(T) == 8) {
+        // FP64, every phase with n = 1: the instantiation without pow
+        if (mode == BMODE_START) z_mode<T, BEH_EP, BMODE_START, true>(g, p, st);
+        else if (mode == BMODE_FINAL) z_mode<T, BEH_EP, BMODE_FINAL, true>(g, p, st);
+        else z_mode<T, BEH_EP, BMODE_ITER, true>(g, p, st);
+    } else if (beh == BEH_EP) {
         if (mode == BMODE_START) z_mode<T, BEH_EP, BMODE_START>(g, p, st);
         else if (mode == BMODE_FINAL) z_mode<T, BEH_EP, BMODE_FINAL>(g, p, st);
         else z_mode<T, BEH_EP, BMODE_ITER>(g, p, st);
@@ -408,8 +413,8 @@ void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st)
     if (!ok) throw std::runtime_error("no basic x kernel for this grid");
 }
 
-template void launch_z<double>(int, int, const Geo&, const BPtr<double>&, hipStream_t);
-template void launch_z<float>(int, int, const Geo&, const BPtr<float>&, hipStream_t);
+template void launch_z<double>(int, int, const Geo&, const BPtr<double>&, hipStream_t, bool);
+template void launch_z<float>(int, int, const Geo&, const BPtr<float>&, hipStream_t, bool);
 template void launch_y12<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y12<float>(const Geo&, const Ptr<float>&, hipStream_t);
 template void launch_y12_inv<double>(const Geo&, const Ptr<double>&, hipStream_t);

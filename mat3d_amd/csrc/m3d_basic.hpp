@@ -80,7 +80,10 @@ __device__ __forceinline__ void iso_C(T la, T mu, const T* d, T* o)
 // elastoPlastic for one voxel.  Returns false when the Newton iteration for
 // the plastic multiplier produces NaN (the reference terminates) or does not
 // reach |res| < 1e-5 within 200 steps.
-template <typename T>
+// LIN: every phase has hardening exponent n = 1, where pow(a, 1) = a and
+// pow(a, 0) = 1 exactly, so the same arithmetic runs without the pow code
+// (and its registers)
+template <typename T, bool LIN = false>
 __device__ __forceinline__ bool j2_update(const double* m, const T* eps, const T* eps_old, const T* sig_old, T ep_eff,
                                           T* sig, T& phi, T& dep, T& seff)
 {
@@ -92,7 +95,11 @@ __device__ __forceinline__ bool j2_update(const double* m, const T* eps, const T
 #pragma unroll
     for (int k = 0; k < 6; ++k) sig[k] = sig_old[k] + c[k];
     seff = vm_eff(sig, sd);
-    phi = seff - Sy - H * pow(ep_eff, n);
+    auto pw = [&](T a, T e) {
+        if constexpr (LIN) return e == T(0) ? T(1) : a; // e is n or n - 1 with n = 1
+        else return pow(a, e);
+    };
+    phi = seff - Sy - H * pw(ep_eff, n);
     if (phi > T(0)) {
         T x = 0, x_o = 0, res;
         bool ok = false;
@@ -108,10 +115,10 @@ __device__ __forceinline__ bool j2_update(const double* m, const T* eps, const T
             // stress; the FP64 polish applies the reference's absolute rule
             const T rtol = sizeof(T) == 4 ? T(1e-6) * (Sy + seff) : T(1e-5);
             for (int iter = 0; iter < 200; ++iter) {
-                x = x_o - (seff - T(3) * mu * x_o - Sy - H * pow(ep_eff + x_o, n)) /
-                              (-T(3) * mu - H * n * pow(ep_eff + x_o, n - T(1)));
+                x = x_o - (seff - T(3) * mu * x_o - Sy - H * pw(ep_eff + x_o, n)) /
+                              (-T(3) * mu - H * n * pw(ep_eff + x_o, n - T(1)));
                 x_o = x;
-                res = seff - T(3) * mu * x - Sy - H * pow(ep_eff + x, n);
+                res = seff - T(3) * mu * x - Sy - H * pw(ep_eff + x, n);
                 if (isnan(x)) break;
                 if (fabs(res) < rtol) { ok = true; break; }
             }

@@ -44,8 +44,10 @@ inline bool supported(int Nx, int Ny, int Nz)
 // of the tiles above, TY, TKx; the y launch takes the 12- or 6-component
 // view of the spectra in p (S: 6-component Green output, K: 12-component)
 // (T = double or float)
+// lin: every phase has hardening exponent n = 1 (FP64 "ep" then runs the
+// return map without pow, bit for bit the same arithmetic)
 template <typename T>
-void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st);
+void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st, bool lin = false);
 template <typename T>
 void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st);
 // inverse y pass of the 12-component spectrum (FP64), ahead of the FINAL z pass

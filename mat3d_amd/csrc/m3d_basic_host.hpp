@@ -31,6 +31,7 @@ struct BasicRunner {
     unsigned char* S12s = nullptr;
     double* psum = nullptr;
     double* psum2 = nullptr; // [NBSUM][BSPLIT] shares of psum
+    bool lin = false;        // every phase has hardening exponent n = 1
     double* pchk = nullptr;
     double* bs = nullptr;
     double* mat = nullptr;
@@ -205,7 +206,7 @@ struct BasicRunner {
         hipStream_t st = e.st;
         BPtr<T> bp = bptr_t<T>();
         Ptr<T> yp = yptr_t<T>();
-        bfast::launch_z<T>(beh, mode == BMODE_ITER ? BMODE_ITER : BMODE_START, gf, bp, st);
+        bfast::launch_z<T>(beh, mode == BMODE_ITER ? BMODE_ITER : BMODE_START, gf, bp, st, lin);
         if (mode == BMODE_START) bfin1<BMODE_START>(gf);
         else if (mode == BMODE_RESTART)
             bfin1<BMODE_RESTART>(gf);
@@ -228,7 +229,7 @@ struct BasicRunner {
         run.done = 0;
         HCHK(hipMemcpyAsync(ctl, &run, sizeof run, hipMemcpyHostToDevice, st));
         bfast::launch_y12_inv<double>(gf, yptr_t<double>(), st);
-        bfast::launch_z<double>(beh, BMODE_FINAL, gf, bptr_t<double>(), st);
+        bfast::launch_z<double>(beh, BMODE_FINAL, gf, bptr_t<double>(), st, lin);
         HCHK(hipGetLastError());
         Ctl after;
         HCHK(hipMemcpyAsync(&after, ctl, sizeof after, hipMemcpyDeviceToHost, st));
@@ -252,6 +253,8 @@ struct BasicRunner {
     {
         hipStream_t st = e.st;
         HCHK(hipMemcpyAsync(mat, mtab.data(), sizeof(double) * mtab.size(), hipMemcpyHostToDevice, st));
+        lin = true;
+        for (size_t k = 4; k < mtab.size(); k += 8) lin = lin && mtab[k] == 1.0;
         Ctl c;
         std::memset(&c, 0, sizeof c);
         double dEs = 0;

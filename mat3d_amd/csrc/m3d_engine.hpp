@@ -1061,7 +1061,27 @@ struct Engine {
                 // if the FP32 phase only stalled on its floored tolerance
                 const bool capped = hctl[s].done != 1;
                 const bool border = hctl[s].dsum < rollback_margin * tol_f && hctl[s].deq < rollback_margin * tol_eq;
-                jr[s] = (capped || border) ? std::max(0, hctl[s].it - rollback) : hctl[s].it;
+                const int it = hctl[s].it;
+                if (capped) {
+                    jr[s] = std::max(0, it - rollback);
+                } else if (border) {
+                    // the reference rule needs Delta-sum < tol at its stop iteration,
+                    // and the FP32 Delta-sum differs from the FP64 one far less than
+                    // rollback_margin: so it can only stop within the trailing run of
+                    // iterations whose FP32 Delta-sum is below rollback_margin x tol.
+                    // Redo that run (at least the switch iterate), at most `rollback`
+                    int k = it;
+                    if (rollback <= 15) {
+                        while (k - 1 > std::max(1, it - rollback) &&
+                               hctl[s].dsum_hist[(k - 1) & 15] < rollback_margin * tol_f)
+                            --k;
+                        jr[s] = std::max(0, k - 1);
+                    } else {
+                        jr[s] = std::max(0, it - rollback);
+                    }
+                } else {
+                    jr[s] = it;
+                }
                 hctl[s].it = jr[s];
                 hctl[s].done = 0;
                 hctl[s].need_cur = hctl[s].need_prev = 0;

@@ -71,6 +71,7 @@ struct Ctl {
     double incNo;     // increment number (1-based)
     double nc;        // N * (C[0]_00/2 + C[1]_00/2): Delta-sum normalisation without sum(E)
     double la0, mu0;  // reference medium (S0 = C0^-1 applied in closed form)
+    double dsum_hist[16]; // dsum of iterate i at [i % 16] (how far an FP32 switch rolls back)
 };
 
 struct Geo {
@@ -820,6 +821,7 @@ __global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double
         const double S = c.corr ? S0 + c.sum_anchor : S0;
         c.it += 1;
         c.dsum = fabs((S - c.sum_prev) / c.norm);
+        c.dsum_hist[c.it & 15] = c.dsum;
         c.sum_prev = S;
         const int need = (c.it > 1) && (fabs(c.dsum) < c.tol_f);
         c.need_cur = need;

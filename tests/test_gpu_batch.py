@@ -182,6 +182,8 @@ def test_specialised_batch_fp32_polish_matches_oracle(probs_fast, oracle_fast):
               f"vs {o['iters']}")
         assert e < 1e-6
         assert all(x["status"] == 1 for x in r["info"])
+        # the FP32 phase's switch and rollback leave the reference's stop iteration unchanged
+        assert [x["iterations"] for x in r["info"]] == o["iters"]
 
 
 def test_full_voronoi_workload_uniform_moduli():

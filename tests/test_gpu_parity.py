@@ -96,6 +96,7 @@ def test_elastic_fp32_polish_meets_bar(n):
     print("fp32+polish", n, "C rel err", err, "iters", its, "oracle", r["iters"],
           "fp64 share", [i["iters_fp64"] for i in s.info])
     assert err < 1e-6
+    assert its == r["iters"]
     eps_o, sig_o = o.fields()
     eps, sig = s.fields(5)
     assert field_rel(sig, sig_o) < 1e-5
