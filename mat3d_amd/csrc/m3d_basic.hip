@@ -35,11 +35,18 @@ struct Comp6 {
 // reads S12): an inverse y pass, then here the c2r and the constitutive
 // update (the history does not change within an increment).  The damage
 // variable is written every iteration: the next one compares with it.
+// The J2 trial stress sigma_old + C (eps - eps_old) is formed, from the
+// start stage on, as sigma_star + C eps with sigma_star = sigma_old - C eps_old
+// written by the start stage: the iterations read 48 bytes of history per
+// voxel instead of 96.
 template <typename T, int BEH, int MODE, int LZ, int TY, bool LIN = false>
 __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 {
     constexpr bool FIN = MODE == BMODE_FINAL;
     constexpr bool STORE = !(MODE == BMODE_ITER && sizeof(T) == 8);
+    // FP64 "ep": the start stage writes sigma_star = sigma_old - C eps_old and
+    // the iterations read it in place of sigma_old and eps_old
+    constexpr bool STAR = sizeof(T) == 8 && BEH == BEH_EP;
     constexpr int C = 12;
     constexpr int W = TY * C;
     constexpr int NZ = 2 * LZ;
@@ -141,9 +148,23 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
             T* sg = s2 + 6 * h;
             if (BEH == BEH_EP) {
                 T so[6], eo[6], phi, dep, seff;
-                fast::vload<T, 6>(p.sig_old + vg * 6, so);
-                fast::vload<T, 6>(p.eps_old + vg * 6, eo);
-                if (!j2_update<T, LIN>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+                if (STAR && MODE != BMODE_START) {
+                    // sigma_star = sigma_old - C eps_old, written by this increment's start
+                    fast::vload<T, 6>(p.sig_star + vg * 6, so);
+                    if (!j2_update<T, LIN, true>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+                } else {
+                    fast::vload<T, 6>(p.sig_old + vg * 6, so);
+                    fast::vload<T, 6>(p.eps_old + vg * 6, eo);
+                    if constexpr (STAR) {
+                        const double la = mt[0], mu = mt[1];
+                        T ce[6], st[6];
+                        iso_C(T(la), T(mu), eo, ce);
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) st[c] = so[c] - ce[c];
+                        fast::vstore<T, 6>(p.sig_star + vg * 6, st);
+                    }
+                    if (!j2_update<T, LIN>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+                }
                 if (STORE) {
                     p.phi[vg] = phi;
                     p.dep[vg] = dep;

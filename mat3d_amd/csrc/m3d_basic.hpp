@@ -42,6 +42,7 @@ struct BPtr {
     T* phi;          // N   yield function of the last constitutive call ("ep")
     T* dep;          // N   plastic multiplier increment ("ep")
     T* seff;         // N   von Mises stress after the return map ("ep")
+    T* sig_star;     // 6N  sigma_old - C eps_old, once per increment ("ep", FP64 specialised kernels)
     long long sU;    // per-solve stride of the 6N fields
     long long sN;    // per-solve stride of the N fields
     cx<T>* S6;       // Green output, 6 components
@@ -83,14 +84,17 @@ __device__ __forceinline__ void iso_C(T la, T mu, const T* d, T* o)
 // LIN: every phase has hardening exponent n = 1, where pow(a, 1) = a and
 // pow(a, 0) = 1 exactly, so the same arithmetic runs without the pow code
 // (and its registers)
-template <typename T, bool LIN = false>
+// STAR: the trial stress sigma_old + C (eps - eps_old) is formed as
+// sig_star + C eps with sig_star = sigma_old - C eps_old kept per increment
+// (passed in sig_old; eps_old unused): one 6-vector read instead of two
+template <typename T, bool LIN = false, bool STAR = false>
 __device__ __forceinline__ bool j2_update(const double* m, const T* eps, const T* eps_old, const T* sig_old, T ep_eff,
                                           T* sig, T& phi, T& dep, T& seff)
 {
     const T la = T(m[0]), mu = T(m[1]), Sy = T(m[2]), H = T(m[3]), n = T(m[4]);
     T d[6], c[6], sd[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) d[k] = eps[k] - eps_old[k];
+    for (int k = 0; k < 6; ++k) d[k] = STAR ? eps[k] : eps[k] - eps_old[k];
     iso_C(la, mu, d, c);
 #pragma unroll
     for (int k = 0; k < 6; ++k) sig[k] = sig_old[k] + c[k];

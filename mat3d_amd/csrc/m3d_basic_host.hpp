@@ -21,7 +21,7 @@ struct BasicRunner {
     bool fast = false; // grid has specialised basic kernels (m3d_basic.hip)
     int beh;
     bool two = false;       // FP32 phase + FP64 polish per increment
-    double* f6 = nullptr;   // eps, sig, sig_old, eps_old (4 x 6N)
+    double* f6 = nullptr;   // eps, sig, sig_old, eps_old, sig_star (5 x 6N)
     double* f1 = nullptr;   // ep|d, phi, dep, seff (4 x N)
     unsigned char* S6 = nullptr;
     unsigned char* S12 = nullptr;
@@ -75,7 +75,7 @@ struct BasicRunner {
         const int nbx = fast ? std::max(g12.nbx, gf.nbx) : g12.nbx;
         const size_t N = (size_t)e.N, H = (size_t)e.H;
         try {
-            e.alloc(&f6, sizeof(double) * 24 * N);
+            e.alloc(&f6, sizeof(double) * 30 * N);
             e.alloc(&f1, sizeof(double) * 4 * N);
             e.alloc(&S6, sizeof(cx<double>) * 12 * H);
             e.alloc(&S12, sizeof(cx<double>) * 12 * H);
@@ -122,6 +122,7 @@ struct BasicRunner {
         p.sig = a6 + 6 * N;
         p.sig_old = a6 + 12 * N;
         p.eps_old = a6 + 18 * N;
+        p.sig_star = sizeof(T) == 8 ? a6 + 24 * N : nullptr;
         p.ep = a1;
         p.phi = a1 + N;
         p.dep = a1 + 2 * N;
