@@ -20,21 +20,25 @@ struct Comp6 {
 };
 
 // ------------------------------------------------------------------ z pass
-// Block: plane x, lines y0..y0+TY-1.  LDS row k (k = 0..LZ) holds W = 12 TY
-// columns, column yl*12 + c: c < 6 sigma, c >= 6 eps.  Slot m of the
-// half-length complex FFT holds voxels (2m, 2m+1) of its line after the c2r.
-//   ITER : eps^ (Green output, 6 comps) -> c2r -> eps; START: eps from HBM
-//   then per voxel the constitutive update (elastoPlastic :640-691 or
-//   elastoDamage :619-638) -> sigma; FP64 partial sums of sigma and eps
-//   (updateE :583-594); r2c of the 12 lines -> [sigma^ | eps^]
-// The FP64 ITER pass keeps eps, sigma and the return-map state (phi,
-// d_ep_p, sigma_eff) out of HBM: only the increment's last iterate needs
-// them (update_old_Variable :693-719, export_homogen_var :722-746), and FINAL
-// recomputes that iterate from the eps^ columns of S12, which the z and y
-// passes of its iteration left transformed along z and y (the x pass only
-// reads S12): an inverse y pass, then here the c2r and the constitutive
-// update (the history does not change within an increment).  The damage
-// variable is written every iteration: the next one compares with it.
+// Block: plane x, lines y0..y0+TY-1.  Slot m of the half-length complex FFT
+// holds voxels (2m, 2m+1) of its line in real space.
+//   START: eps from HBM, the constitutive update (elastoPlastic :640-691 or
+//     elastoDamage :619-638) -> sigma, r2c of the 12 lines -> [sigma^ | eps^]
+//     in S12 (12 components per kz); LDS row k holds W = 12 TY columns,
+//     column yl*12 + c: c < 6 sigma, c >= 6 eps.
+//   ITER: eps^ (Green output, 6 comps, S6) -> c2r -> eps -> sigma, written
+//     over eps in LDS (W = 6 TY columns, column yl*6 + c), r2c of the 6
+//     lines -> sigma^ in S12 with 6 components per kz.  eps^ itself is not
+//     transformed again: the x pass keeps the spectrum of the strain it
+//     produced (the forward transform of the c2r of a Hermitian spectrum is
+//     that spectrum), so FFT_epsilon (:772) costs nothing here.
+//   FINAL: the ITER pass of the increment's last iterate without the r2c,
+//     storing eps, sigma and the return-map state: the FP64 ITER pass keeps
+//     them out of HBM (only that iterate needs them, update_old_Variable
+//     :693-719, export_homogen_var :722-746), and its Green output is still
+//     in the S6 buffer it was read from (the x pass writes the other one).
+// All modes: FP64 partial sums of sigma and eps (updateE :583-594).  The
+// damage variable is written every iteration: the next one compares with it.
 // The J2 trial stress sigma_old + C (eps - eps_old) is formed, from the
 // start stage on, as sigma_star + C eps with sigma_star = sigma_old - C eps_old
 // written by the start stage: the iterations read 48 bytes of history per
@@ -43,11 +47,13 @@ template <typename T, int BEH, int MODE, int LZ, int TY, bool LIN = false>
 __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 {
     constexpr bool FIN = MODE == BMODE_FINAL;
+    constexpr bool START = MODE == BMODE_START;
     constexpr bool STORE = !(MODE == BMODE_ITER && sizeof(T) == 8);
     // FP64 "ep": the start stage writes sigma_star = sigma_old - C eps_old and
     // the iterations read it in place of sigma_old and eps_old
     constexpr bool STAR = sizeof(T) == 8 && BEH == BEH_EP;
-    constexpr int C = 12;
+    constexpr int C = START ? 12 : 6; // LDS columns per line
+    constexpr int EO = START ? 6 : 0; // column of eps_0 within a line
     constexpr int W = TY * C;
     constexpr int NZ = 2 * LZ;
     constexpr int NZH = LZ + 1;
@@ -71,31 +77,20 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     const uint8_t* ph = p.ph + vox0;
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
 
-    if (MODE == BMODE_ITER || FIN) {
-        if (FIN) {
-            const cx<T>* src = p.S12 + ((long long)x * g.Ny + y0) * NZH * 12;
-            for (int e = tid; e < TY * NZH * 6; e += NT) {
-                const int yl = e / (NZH * 6);
-                const int r = e - yl * (NZH * 6);
-                const int kz = r / 6, c = r - kz * 6;
-                sl[kz * W + yl * C + 6 + c] = src[((long long)yl * NZH + kz) * 12 + 6 + c];
-            }
-        } else {
-            const cx<T>* src = p.S6 + ((long long)x * g.Ny + y0) * NZH * 6;
-            for (int e = tid; e < TY * NZH * 6; e += NT) {
-                const int yl = e / (NZH * 6);
-                const int r = e - yl * (NZH * 6);
-                const int kz = r / 6, c = r - kz * 6;
-                sl[kz * W + yl * C + 6 + c] = src[e];
-            }
+    if constexpr (!START) {
+        // the TY lines of eps^ are contiguous in S6: [yl][kz][c]
+        const cx<T>* src = p.S6 + ((long long)x * g.Ny + y0) * NZH * 6;
+        for (int e = tid; e < TY * NZH * 6; e += NT) {
+            const int yl = e / (NZH * 6);
+            const int r = e - yl * (NZH * 6);
+            const int kz = r / 6, c = r - kz * 6;
+            sl[kz * W + yl * 6 + c] = src[e];
         }
         __syncthreads();
         // c2r pre-processing of the eps lines; kz = 0 and Nz/2 keep real parts
         constexpr int half = LZ / 2 + 1;
-        constexpr int W6 = TY * 6;
-        for (int e = tid; e < W6 * half; e += NT) {
-            const int li = e % W6, k = e / W6;
-            const int l = (li / 6) * C + 6 + li % 6;
+        for (int e = tid; e < W * half; e += NT) {
+            const int l = e % W, k = e / W;
             if (k == 0) {
                 const T x0 = sl[l].re, xl = sl[LZ * W + l].re;
                 sl[l] = cx<T>{x0 + xl, x0 - xl};
@@ -113,7 +108,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
             }
         }
         __syncthreads();
-        fast::fft_sub<T, LZ, W6, NT, 1, 2, W, 6, C, 6>(sl, tw);
+        fast::fft<T, LZ, W, NT, 1, 2>(sl, tw);
     } else {
         __syncthreads();
     }
@@ -123,18 +118,16 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 #pragma unroll
     for (int q = 0; q < NBSUM; ++q) acc[q] = 0.0;
     bool bad = false;
-    // FINAL: the inverse y and z transforms of the unnormalised forward ones
-    const T fsc = FIN ? T(1.0 / ((double)g.Ny * (double)g.Nz)) : T(1);
     for (int q = tid; q < TY * LZ; q += NT) {
         const int yl = q / LZ, m = q - yl * LZ;
         const int v0 = yl * NZ + 2 * m;
         T e2[12], s2[12];
-        if (MODE == BMODE_ITER || FIN) {
+        if constexpr (!START) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-                const cx<T> u = sl[m * W + yl * C + 6 + c];
-                e2[c] = FIN ? u.re * fsc : u.re;
-                e2[6 + c] = FIN ? u.im * fsc : u.im;
+                const cx<T> u = sl[m * W + yl * C + c];
+                e2[c] = u.re;
+                e2[6 + c] = u.im;
             }
             if (STORE) fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
         } else {
@@ -148,7 +141,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
             T* sg = s2 + 6 * h;
             if (BEH == BEH_EP) {
                 T so[6], eo[6], phi, dep, seff;
-                if (STAR && MODE != BMODE_START) {
+                if (STAR && !START) {
                     // sigma_star = sigma_old - C eps_old, written by this increment's start
                     fast::vload<T, 6>(p.sig_star + vg * 6, so);
                     if (!j2_update<T, LIN, true>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
@@ -183,10 +176,11 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         }
         if (STORE) fast::vstore<T, 12>(sig + (long long)v0 * 6, s2);
         if (FIN) continue;
+        // sigma over the slots this thread read eps from (ITER), or beside eps (START)
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             sl[m * W + yl * C + c] = cx<T>{s2[c], s2[6 + c]};
-            if (MODE == BMODE_START) sl[m * W + yl * C + 6 + c] = cx<T>{e2[c], e2[6 + c]};
+            if (START) sl[m * W + yl * C + EO + c] = cx<T>{e2[c], e2[6 + c]};
         }
     }
     if (bad) ctl.err = 1;
@@ -209,7 +203,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     }
     __syncthreads();
 
-    // r2c of the 12 lines
+    // r2c of the W lines
     fast::fft<T, LZ, W, NT, -1, 2>(sl, tw);
     {
         constexpr int half = LZ / 2 + 1;
@@ -240,15 +234,23 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 }
 
 // ------------------------------------------------------------------ x pass
-// Block: row y, kz tile [kz0, kz0+nk); LDS row kx holds W = 12 TK columns,
-// column kzl*12 + c.  Forward x FFT of the 12 lines, eqVar partials from
-// sigma^ (convergenceCheck :351-389), Green step t = eps^ - Gamma0:sigma^
-// (multiplyGF :323-338, DC = E) written over the sigma columns, inverse x
-// FFT of those 6 columns per kz.
-template <typename T, int L, int TK>
+// Block: row y, kz tile [kz0, kz0+nk).  Forward x FFT of the sigma^ lines,
+// eqVar partials from sigma^ (convergenceCheck :351-389), Green step
+// eps^ - Gamma0:sigma^ (multiplyGF :323-338, DC = E), inverse x FFT of its
+// 6 lines per kz into the S6 buffer p.S6w.
+//   E6 = false (after a START z pass): sigma^ and eps^ both come from S12
+//     (12 components per kz) and both are x-transformed here; LDS row kx
+//     holds W = 12 TK columns, column kzl*12 + c (c >= 6 eps).
+//   E6 = true (after an ITER z pass): sigma^ comes from S12 with 6
+//     components per kz (LDS row kx: W = 6 TK columns, kzl*6 + c); eps^ is
+//     read from Eh, where the previous x pass left it already x-transformed,
+//     by the thread that applies the Green step to it.
+// Both write the new eps^ (unnormalised, before the inverse x FFT) to Eh,
+// one contiguous [kx][kzl][c] block per tile, for the next iteration.
+template <typename T, int L, int TK, bool E6>
 __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
 {
-    constexpr int C = 12;
+    constexpr int C = E6 ? 6 : 12; // LDS columns per kz
     constexpr int W = TK * C;
     constexpr int NT = XNT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -264,19 +266,22 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
     if (ctl.done || y >= g.Ny) return;
     const int nk = min(TK, g.Nzh - kz0);
     const int tid = threadIdx.x;
+    cx<T>* eh = p.Eh + (long long)t * (L * TK * 6);
+    for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
     const cx<T>* base = p.S12 + ((long long)y * g.Nzh + kz0) * C;
     const long long xstride = (long long)g.Ny * g.Nzh * C;
-    for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
     fast::rows_to_lds<T, L, W, NT, sizeof(T) == 4>(sl, base, fast::uniform_rows(xstride), nk * C);
     __syncthreads();
     fast::fft<T, L, W, NT, -1, 1>(sl, tw);
 
     const T la0 = T(ctl.la0), mu0 = T(ctl.mu0);
+    const T sc = T(g.invN);
     double acc = 0.0;
     for (int f = tid; f < L * nk; f += NT) {
         const int kx = f / nk, kzl = f - kx * nk;
         const int kz = kz0 + kzl;
         cx<T>* v = sl + kx * W + kzl * C;
+        cx<T>* eo = eh + (long long)f * 6;
         double a2[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) a2[c] = (double)v[c].re * (double)v[c].re + (double)v[c].im * (double)v[c].im;
@@ -289,8 +294,14 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
         if (kx == 0 && y == 0 && kz == 0) {
             p.pchk[g.nbx] = sqrt(a2[0] + a2[1] + a2[2] + 2 * a2[3] + 2 * a2[4] + 2 * a2[5]);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] = cx<T>{T(ctl.E[c]), T(0)};
+            for (int c = 0; c < 6; ++c) {
+                v[c] = cx<T>{T(ctl.E[c]), T(0)};
+                eo[c] = cx<T>{T(ctl.E[c] / g.invN), T(0)};
+            }
         } else {
+            cx<T> ev[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ev[c] = E6 ? eo[c] : v[6 + c];
             const int fz = Elastic::fmap(kz, g.Nz), fy = Elastic::fmap(y, g.Ny), fxx = Elastic::fmap(kx, g.Nx);
             T xi[3] = {T(fz), T(fy), T(fxx)};
             cx<T> sg[6], o[6];
@@ -309,21 +320,32 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
                     for (int c = 0; c < 6; ++c) o[c] = cscale(o[c] + o2[c], T(0.5));
                 }
             }
-            const T sc = T(g.invN);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] = cscale(v[6 + c] - o[c], sc);
+            for (int c = 0; c < 6; ++c) {
+                const cx<T> n = ev[c] - o[c];
+                eo[c] = n;
+                v[c] = cscale(n, sc);
+            }
         }
     }
     const double bs = block_sum(acc, red);
     if (tid == 0) p.pchk[t] = bs;
     __syncthreads();
-    fast::fft_sub<T, L, TK * 6, NT, 1, 1, W, 6, C, 0>(sl, tw);
-    cx<T>* ob = p.S6 + ((long long)y * g.Nzh + kz0) * 6;
+    cx<T>* ob = p.S6w + ((long long)y * g.Nzh + kz0) * 6;
     const long long ostride = (long long)g.Ny * g.Nzh * 6;
-    for (int e = tid; e < L * nk * 6; e += NT) {
-        const int xx = e / (nk * 6), r = e - xx * (nk * 6);
-        const int kzl = r / 6, c = r - kzl * 6;
-        ob[xx * ostride + r] = sl[xx * W + kzl * C + c];
+    if constexpr (E6) {
+        fast::fft<T, L, W, NT, 1, 1>(sl, tw);
+        for (int e = tid; e < L * nk * 6; e += NT) {
+            const int xx = e / (nk * 6), r = e - xx * (nk * 6);
+            ob[xx * ostride + r] = sl[xx * W + r];
+        }
+    } else {
+        fast::fft_sub<T, L, TK * 6, NT, 1, 1, W, 6, C, 0>(sl, tw);
+        for (int e = tid; e < L * nk * 6; e += NT) {
+            const int xx = e / (nk * 6), r = e - xx * (nk * 6);
+            const int kzl = r / 6, c = r - kzl * 6;
+            ob[xx * ostride + r] = sl[xx * W + kzl * C + c];
+        }
     }
 }
 
@@ -336,7 +358,8 @@ bool z_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
     if (g.Nz != 2 * LZ) return false;
     constexpr int TY = zlines(LZ, 2 * LZ);
     if (g.TY != TY) throw std::runtime_error("basic z tile mismatch");
-    const size_t sm = ((size_t)(LZ + 1) * TY * 12 + 2 * LZ) * sizeof(cx<T>);
+    constexpr int C = MODE == BMODE_START ? 12 : 6;
+    const size_t sm = ((size_t)(LZ + 1) * TY * C + 2 * LZ) * sizeof(cx<T>);
     hipLaunchKernelGGL((kzb<T, BEH, MODE, LZ, TY, LIN>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
     return true;
 }
@@ -373,18 +396,27 @@ void y_run(const Geo& g, const Ptr<T>& p, hipStream_t st)
     if (!ok) throw std::runtime_error("no basic y kernel for this grid");
 }
 
-template <typename T, int L>
+template <typename T, int L, bool E6>
 bool x_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
 {
     if (g.Nx != L) return false;
     constexpr int TK = xtile(L);
     if (g.TKx != TK) throw std::runtime_error("basic x tile mismatch");
-    const size_t sm = ((size_t)L * TK * 12 + L) * sizeof(cx<T>);
+    const size_t sm = ((size_t)L * TK * (E6 ? 6 : 12) + L) * sizeof(cx<T>);
     if (sm > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kxb<T, L, TK>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kxb<T, L, TK, E6>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    hipLaunchKernelGGL((kxb<T, L, TK>), dim3(g.nbx), dim3(XNT), sm, st, g, p);
+    hipLaunchKernelGGL((kxb<T, L, TK, E6>), dim3(g.nbx), dim3(XNT), sm, st, g, p);
     return true;
+}
+
+template <typename T, bool E6>
+void x_run(const Geo& g, const BPtr<T>& p, hipStream_t st)
+{
+    const bool ok = x_at<T, 256, E6>(g, p, st) || x_at<T, 192, E6>(g, p, st) || x_at<T, 128, E6>(g, p, st) ||
+                    x_at<T, 96, E6>(g, p, st) || x_at<T, 64, E6>(g, p, st) || x_at<T, 48, E6>(g, p, st) ||
+                    x_at<T, 32, E6>(g, p, st);
+    if (!ok) throw std::runtime_error("no basic x kernel for this grid");
 }
 
 } // namespace
@@ -415,9 +447,9 @@ void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st)
 }
 
 template <typename T>
-void launch_y12_inv(const Geo& g, const Ptr<T>& p, hipStream_t st)
+void launch_y6_fwd(const Geo& g, const Ptr<T>& p, hipStream_t st)
 {
-    y_run<T, Comp12, 1, BUF_K>(g, p, st);
+    y_run<T, Comp6, -1, BUF_K>(g, p, st);
 }
 
 template <typename T>
@@ -427,22 +459,22 @@ void launch_y6(const Geo& g, const Ptr<T>& p, hipStream_t st)
 }
 
 template <typename T>
-void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st)
+void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st, bool e6)
 {
-    const bool ok = x_at<T, 256>(g, p, st) || x_at<T, 192>(g, p, st) || x_at<T, 128>(g, p, st) ||
-                    x_at<T, 96>(g, p, st) || x_at<T, 64>(g, p, st) || x_at<T, 48>(g, p, st) || x_at<T, 32>(g, p, st);
-    if (!ok) throw std::runtime_error("no basic x kernel for this grid");
+    if (e6) x_run<T, true>(g, p, st);
+    else x_run<T, false>(g, p, st);
 }
 
 template void launch_z<double>(int, int, const Geo&, const BPtr<double>&, hipStream_t, bool);
 template void launch_z<float>(int, int, const Geo&, const BPtr<float>&, hipStream_t, bool);
 template void launch_y12<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y12<float>(const Geo&, const Ptr<float>&, hipStream_t);
-template void launch_y12_inv<double>(const Geo&, const Ptr<double>&, hipStream_t);
+template void launch_y6_fwd<double>(const Geo&, const Ptr<double>&, hipStream_t);
+template void launch_y6_fwd<float>(const Geo&, const Ptr<float>&, hipStream_t);
 template void launch_y6<double>(const Geo&, const Ptr<double>&, hipStream_t);
 template void launch_y6<float>(const Geo&, const Ptr<float>&, hipStream_t);
-template void launch_x<double>(const Geo&, const BPtr<double>&, hipStream_t);
-template void launch_x<float>(const Geo&, const BPtr<float>&, hipStream_t);
+template void launch_x<double>(const Geo&, const BPtr<double>&, hipStream_t, bool);
+template void launch_x<float>(const Geo&, const BPtr<float>&, hipStream_t, bool);
 
 } // namespace bfast
 } // namespace m3d

@@ -27,8 +27,8 @@ namespace m3d {
 
 enum { BEH_EP = 1, BEH_ED = 2 };
 // FINAL (specialised kernels only): the fields and state of the last iterate
-// of an increment, recomputed from the [sigma^ | eps^] spectrum its z and y
-// passes left behind (the FP64 iterations do not write them)
+// of an increment, recomputed from the Green output its z pass read (the
+// FP64 iterations do not write them)
 enum { BMODE_START = 0, BMODE_ITER = 1, BMODE_RESTART = 2, BMODE_FINAL = 3 };
 constexpr int NBSUM = 12; // sums per solve in the z pass: sigma_c, eps_c
 
@@ -45,8 +45,11 @@ struct BPtr {
     T* sig_star;     // 6N  sigma_old - C eps_old, once per increment ("ep", FP64 specialised kernels)
     long long sU;    // per-solve stride of the 6N fields
     long long sN;    // per-solve stride of the N fields
-    cx<T>* S6;       // Green output, 6 components
-    cx<T>* S12;      // [sigma^ | eps^], 12 components
+    cx<T>* S6;       // Green output, 6 components (read by the z pass)
+    cx<T>* S12;      // [sigma^ | eps^], 12 components (generic kernels, START)
+                     // or sigma^, 6 components (specialised ITER)
+    cx<T>* S6w;      // Green output written by the specialised x pass (the other S6 buffer)
+    cx<T>* Eh;       // eps^ after the Green step, tile order of the specialised x pass
     long long sS6, sS12;
     const uint8_t* ph;
     const double* mat; // per phase {la, mu, Sy, H, n, P, Y0, 0}

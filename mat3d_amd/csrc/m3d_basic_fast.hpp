@@ -11,14 +11,20 @@
 namespace m3d {
 namespace bfast {
 
-constexpr int ZNT = 128; // threads of the z pass
-constexpr int XNT = 256; // threads of the x pass
+#ifndef M3D_BZ_THREADS
+#define M3D_BZ_THREADS 256
+#endif
+#ifndef M3D_BZ_TILE
+#define M3D_BZ_TILE 2304
+#endif
+constexpr int ZNT = M3D_BZ_THREADS; // threads of the z pass
+constexpr int XNT = 256;            // threads of the x pass
 
-// z tile: TY lines (y) of 12 components each, TY*12*LZ <= 2304 complex
+// z tile: TY lines (y) of 12 components each, TY*12*LZ <= M3D_BZ_TILE complex
 constexpr int zlines(int LZ, int Ny)
 {
     int ty = 1;
-    while (ty < 8 && (ty * 2) * 12 * LZ <= 2304 && Ny % (ty * 2) == 0) ty *= 2;
+    while (ty < 8 && (ty * 2) * 12 * LZ <= M3D_BZ_TILE && Ny % (ty * 2) == 0) ty *= 2;
     return ty;
 }
 // x tile: TK kz values of 12 components
@@ -50,11 +56,13 @@ template <typename T>
 void launch_z(int beh, int mode, const Geo& g, const BPtr<T>& p, hipStream_t st, bool lin = false);
 template <typename T>
 void launch_y12(const Geo& g, const Ptr<T>& p, hipStream_t st);
-// inverse y pass of the 12-component spectrum (FP64), ahead of the FINAL z pass
+// forward y pass of the 6-component sigma^ spectrum that an ITER z pass
+// leaves in the K view
 template <typename T>
-void launch_y12_inv(const Geo& g, const Ptr<T>& p, hipStream_t st);
+void launch_y6_fwd(const Geo& g, const Ptr<T>& p, hipStream_t st);
+// e6: the x pass after an ITER z pass (sigma^ with 6 components, eps^ from Eh)
 template <typename T>
-void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st);
+void launch_x(const Geo& g, const BPtr<T>& p, hipStream_t st, bool e6);
 template <typename T>
 void launch_y6(const Geo& g, const Ptr<T>& p, hipStream_t st);
 

@@ -23,12 +23,15 @@ struct BasicRunner {
     bool two = false;       // FP32 phase + FP64 polish per increment
     double* f6 = nullptr;   // eps, sig, sig_old, eps_old, sig_star (5 x 6N)
     double* f1 = nullptr;   // ep|d, phi, dep, seff (4 x N)
-    unsigned char* S6 = nullptr;
+    unsigned char* S6 = nullptr;  // two 6-component spectra: the specialised passes alternate them
     unsigned char* S12 = nullptr;
+    unsigned char* Eh = nullptr;  // eps^ of the specialised x pass
     float* f6s = nullptr;   // FP32 phase: the same fields in FP32
     float* f1s = nullptr;
     unsigned char* S6s = nullptr;
     unsigned char* S12s = nullptr;
+    unsigned char* Ehs = nullptr;
+    size_t neh = 0;         // complex values of Eh
     double* psum = nullptr;
     double* psum2 = nullptr; // [NBSUM][BSPLIT] shares of psum
     bool lin = false;        // every phase has hardening exponent n = 1
@@ -79,6 +82,10 @@ struct BasicRunner {
             e.alloc(&f1, sizeof(double) * 4 * N);
             e.alloc(&S6, sizeof(cx<double>) * 12 * H);
             e.alloc(&S12, sizeof(cx<double>) * 12 * H);
+            if (fast) {
+                neh = (size_t)gf.nbx * e.Nx * gf.TKx * 6;
+                e.alloc(&Eh, sizeof(cx<double>) * neh);
+            }
             e.alloc(&psum, sizeof(double) * NBSUM * nbz);
             e.alloc(&psum2, sizeof(double) * NBSUM * BSPLIT);
             e.alloc(&pchk, sizeof(double) * (nbx + 1));
@@ -91,6 +98,7 @@ struct BasicRunner {
                 e.alloc(&f1s, sizeof(float) * 4 * N);
                 e.alloc(&S6s, sizeof(cx<float>) * 12 * H);
                 e.alloc(&S12s, sizeof(cx<float>) * 12 * H);
+                e.alloc(&Ehs, sizeof(cx<float>) * neh);
             }
         } catch (...) {
             release();
@@ -99,19 +107,29 @@ struct BasicRunner {
     }
     void release()
     {
-        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)psum, (void*)psum2, (void*)pchk, (void*)bs, (void*)mat,
-                        (void*)ctl, (void*)f6s, (void*)f1s, (void*)S6s, (void*)S12s})
+        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)Eh, (void*)psum, (void*)psum2, (void*)pchk,
+                        (void*)bs, (void*)mat, (void*)ctl, (void*)f6s, (void*)f1s, (void*)S6s, (void*)S12s, (void*)Ehs})
             if (q) (void)hipFree(q);
         f6 = f1 = psum = psum2 = pchk = bs = mat = nullptr;
-        S6 = S12 = nullptr;
+        S6 = S12 = Eh = nullptr;
         f6s = f1s = nullptr;
-        S6s = S12s = nullptr;
+        S6s = S12s = Ehs = nullptr;
         ctl = nullptr;
     }
     ~BasicRunner() { release(); }
 
+    // the S6 buffer `half` (0 or 1) of precision T
     template <typename T>
-    BPtr<T> bptr_t()
+    cx<T>* s6_of(int half)
+    {
+        cx<T>* b = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
+        return b + (half ? 6 * e.H : 0);
+    }
+    // par < 0: the generic kernels (one S6 buffer, read and written in place);
+    // par = 0 / 1: the specialised z pass reads S6 buffer par and the x pass
+    // writes buffer 1 - par
+    template <typename T>
+    BPtr<T> bptr_t(int par = -1)
     {
         BPtr<T> p;
         std::memset(&p, 0, sizeof p);
@@ -132,6 +150,11 @@ struct BasicRunner {
         p.S6 = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
         p.S12 = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S12 : S12s);
         p.sS6 = p.sS12 = 12 * e.H;
+        if (par >= 0) {
+            p.S6 = s6_of<T>(par);
+            p.S6w = s6_of<T>(1 - par);
+            p.Eh = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? Eh : Ehs);
+        }
         p.ph = e.ph;
         p.mat = mat;
         const cx<T>* const* tw = e.template twiddles_of<T>();
@@ -144,11 +167,11 @@ struct BasicRunner {
         return p;
     }
     template <typename T>
-    Ptr<T> yptr_t()
+    Ptr<T> yptr_t(int par = -1)
     {
         Ptr<T> p;
         std::memset(&p, 0, sizeof p);
-        p.S = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
+        p.S = par >= 0 ? s6_of<T>(1 - par) : reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S6 : S6s);
         p.K = reinterpret_cast<cx<T>*>(sizeof(T) == 8 ? S12 : S12s);
         p.sS = 12 * e.H;
         const cx<T>* const* tw = e.template twiddles_of<T>();
@@ -199,38 +222,44 @@ struct BasicRunner {
         hipLaunchKernelGGL((k_ypass<double, 6, 1, BUF_MAIN, GEN>), dim3(g6.Nx * g6.nky), dim3(XYTHREADS), ysm6, st, g6, yp);
     }
 
-    // one stage with the specialised kernels (same sequence as stage()); the
-    // restart stage is a start stage (strain from HBM) that keeps itNo
+    // one stage with the specialised kernels (the sequence of stage()); the
+    // restart stage is a start stage (strain from HBM) that keeps itNo.  A
+    // start stage transforms [sigma | eps] and writes S6 buffer 0 (par = 1);
+    // an iteration reads S6 buffer par, transforms sigma only (the x pass
+    // keeps eps^ in Eh) and writes buffer 1 - par.
     template <typename T = double>
-    void stage_fast(int mode)
+    void stage_fast(int mode, int par)
     {
         hipStream_t st = e.st;
-        BPtr<T> bp = bptr_t<T>();
-        Ptr<T> yp = yptr_t<T>();
-        bfast::launch_z<T>(beh, mode == BMODE_ITER ? BMODE_ITER : BMODE_START, gf, bp, st, lin);
+        const bool iter = mode == BMODE_ITER;
+        BPtr<T> bp = bptr_t<T>(par);
+        Ptr<T> yp = yptr_t<T>(par);
+        bfast::launch_z<T>(beh, iter ? BMODE_ITER : BMODE_START, gf, bp, st, lin);
         if (mode == BMODE_START) bfin1<BMODE_START>(gf);
         else if (mode == BMODE_RESTART)
             bfin1<BMODE_RESTART>(gf);
         else bfin1<BMODE_ITER>(gf);
-        bfast::launch_y12<T>(gf, yp, st);
-        bfast::launch_x<T>(gf, bp, st);
+        if (iter) bfast::launch_y6_fwd<T>(gf6, yp, st);
+        else bfast::launch_y12<T>(gf, yp, st);
+        bfast::launch_x<T>(gf, bp, st, iter);
         hipLaunchKernelGGL((k_bfin2<0>), dim3(1), dim3(FTHREADS), 0, st, gf, ctl, pchk);
         bfast::launch_y6<T>(gf6, yp, st);
     }
 
     // the fields and state of the increment's last iterate on the
-    // specialised FP64 path, whose iterations do not store them: inverse y
-    // pass of the [sigma^ | eps^] spectrum, then the z pass in FINAL mode.
-    // Both passes skip a solve marked done, so the flag is cleared around
-    // them; c is the host copy of the control block after the iterations.
-    void final_fast(Ctl& c)
+    // specialised FP64 path, whose iterations do not store them: the z pass
+    // in FINAL mode on the S6 buffer that iterate was computed from.  The
+    // iterations after the start stage (itNo it0) alternate the buffers from
+    // buffer 0, so iterate itNo c.it read buffer (c.it - it0 - 1) & 1.  The
+    // z pass skips a solve marked done, so the flag is cleared around it; c
+    // is the host copy of the control block after the iterations.
+    void final_fast(Ctl& c, int it0)
     {
         hipStream_t st = e.st;
         Ctl run = c;
         run.done = 0;
         HCHK(hipMemcpyAsync(ctl, &run, sizeof run, hipMemcpyHostToDevice, st));
-        bfast::launch_y12_inv<double>(gf, yptr_t<double>(), st);
-        bfast::launch_z<double>(beh, BMODE_FINAL, gf, bptr_t<double>(), st, lin);
+        bfast::launch_z<double>(beh, BMODE_FINAL, gf, bptr_t<double>((c.it - it0 - 1) & 1), st, lin);
         HCHK(hipGetLastError());
         Ctl after;
         HCHK(hipMemcpyAsync(&after, ctl, sizeof after, hipMemcpyDeviceToHost, st));
@@ -240,9 +269,9 @@ struct BasicRunner {
         HCHK(hipStreamSynchronize(st));
     }
 
-    void run_stage(int mode)
+    void run_stage(int mode, int par)
     {
-        if (fast) stage_fast(mode);
+        if (fast) stage_fast(mode, par);
         else if (g12.gen) stage<true>(mode);
         else stage<false>(mode);
     }
@@ -298,8 +327,8 @@ struct BasicRunner {
         auto capture = [&](GraphGuard& gg, bool fp32) {
             HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < ck; ++k) {
-                if (fp32) stage_fast<float>(BMODE_ITER);
-                else run_stage(BMODE_ITER);
+                if (fp32) stage_fast<float>(BMODE_ITER, k & 1);
+                else run_stage(BMODE_ITER, k & 1);
             }
             HCHK(hipStreamEndCapture(st, &gg.g));
             hipError_t ge = hipGraphInstantiate(&gg.x, gg.g, nullptr, nullptr, 0);
@@ -335,7 +364,7 @@ struct BasicRunner {
                 HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
                 hipLaunchKernelGGL((k_bcopy<double, float>), dim3(nb), dim3(256), 0, st, g12, bp, bp32,
                                    BC_EPS | BC_SIGOLD | BC_EPSOLD | BC_EP);
-                stage_fast<float>(BMODE_START);
+                stage_fast<float>(BMODE_START, 1);
                 HCHK(hipGetLastError());
                 replay(g32.x);
                 if (c.done == 2) {
@@ -354,11 +383,12 @@ struct BasicRunner {
                     c.done = 0;
                     c.tol_f = tol_f;
                     c.tol_eq = tol_eq;
+                    const int it0 = c.it;
                     HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
-                    stage_fast<double>(BMODE_RESTART);
+                    stage_fast<double>(BMODE_RESTART, 1);
                     HCHK(hipGetLastError());
                     replay(g64.x);
-                    if (c.done != 3) final_fast(c);
+                    if (c.done != 3) final_fast(c, it0);
                     fp64_run = false;
                 } else {
                     // the FP32 return map failed: redo the increment in FP64 (its
@@ -372,10 +402,10 @@ struct BasicRunner {
                 c.tol_f = tol_f;
                 c.tol_eq = tol_eq;
                 HCHK(hipMemcpyAsync(ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
-                run_stage(BMODE_START);
+                run_stage(BMODE_START, 1);
                 HCHK(hipGetLastError());
                 replay(g64.x);
-                if (fast && c.done != 3) final_fast(c);
+                if (fast && c.done != 3) final_fast(c, 0);
             }
             if (c.done == 3)
                 throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};
