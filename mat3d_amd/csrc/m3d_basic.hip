@@ -76,6 +76,21 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     T* sig = p.sig + vox0 * 6;
     const uint8_t* ph = p.ph + vox0;
     for (int i = tid; i < NZ; i += NT) tw[i] = p.twz[i];
+    // one voxel pair per thread; the iterations (PRE) load the pair's J2
+    // history and phases before the transforms, so that they arrive during them
+    static_assert(TY * LZ <= NT, "one voxel pair per thread");
+    constexpr bool PRE = STAR && !START;
+    T hs[12], hep[2];
+    int hph[2] = {0, 0};
+    if (PRE && tid < TY * LZ) {
+        const int yl = tid / LZ, m = tid - yl * LZ;
+        const int v0 = yl * NZ + 2 * m;
+        fast::vload<T, 12>(p.sig_star + (vox0 + v0) * 6, hs);
+        hep[0] = p.ep[vox0 + v0];
+        hep[1] = p.ep[vox0 + v0 + 1];
+        hph[0] = ph[v0];
+        hph[1] = ph[v0 + 1];
+    }
 
     if constexpr (!START) {
         // the TY lines of eps^ are contiguous in S6: [yl][kz][c]
@@ -136,15 +151,16 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const long long vg = vox0 + v0 + h;
-            const double* mt = p.mat + ph[v0 + h] * 8;
+            const double* mt = p.mat + (PRE ? hph[h] : ph[v0 + h]) * 8;
             T* e = e2 + 6 * h;
             T* sg = s2 + 6 * h;
             if (BEH == BEH_EP) {
                 T so[6], eo[6], phi, dep, seff;
-                if (STAR && !START) {
+                if constexpr (PRE) {
                     // sigma_star = sigma_old - C eps_old, written by this increment's start
-                    fast::vload<T, 6>(p.sig_star + vg * 6, so);
-                    if (!j2_update<T, LIN, true>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) so[c] = hs[6 * h + c];
+                    if (!j2_update<T, LIN, true>(mt, e, eo, so, hep[h], sg, phi, dep, seff)) bad = true;
                 } else {
                     fast::vload<T, 6>(p.sig_old + vg * 6, so);
                     fast::vload<T, 6>(p.eps_old + vg * 6, eo);
@@ -244,7 +260,9 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 //   E6 = true (after an ITER z pass): sigma^ comes from S12 with 6
 //     components per kz (LDS row kx: W = 6 TK columns, kzl*6 + c); eps^ is
 //     read from Eh, where the previous x pass left it already x-transformed,
-//     by the thread that applies the Green step to it.
+//     by the thread that applies the Green step to it (L*TK <= XNT: one
+//     frequency per thread), issued with the sigma^ loads so that it arrives
+//     during the forward FFT.
 // Both write the new eps^ (unnormalised, before the inverse x FFT) to Eh,
 // one contiguous [kx][kzl][c] block per tile, for the next iteration.
 template <typename T, int L, int TK, bool E6>
@@ -267,10 +285,21 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
     const int nk = min(TK, g.Nzh - kz0);
     const int tid = threadIdx.x;
     cx<T>* eh = p.Eh + (long long)t * (L * TK * 6);
+    static_assert(L * TK <= NT, "one frequency per thread");
     for (int i = tid; i < L; i += NT) tw[i] = p.twx[i];
     const cx<T>* base = p.S12 + ((long long)y * g.Nzh + kz0) * C;
     const long long xstride = (long long)g.Ny * g.Nzh * C;
-    fast::rows_to_lds<T, L, W, NT, sizeof(T) == 4>(sl, base, fast::uniform_rows(xstride), nk * C);
+    cx<T> ev[6];
+    {
+        using RT = fast::RowTile<T, L, W, NT, sizeof(T) == 4>;
+        typename RT::V r[RT::PER];
+        fast::rows_load<T, L, W, NT, sizeof(T) == 4>(r, base, fast::uniform_rows(xstride), nk * C);
+        if (E6 && tid < L * nk) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ev[c] = eh[(long long)tid * 6 + c];
+        }
+        fast::rows_commit<T, L, W, NT, sizeof(T) == 4>(sl, r);
+    }
     __syncthreads();
     fast::fft<T, L, W, NT, -1, 1>(sl, tw);
 
@@ -299,9 +328,10 @@ __global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
                 eo[c] = cx<T>{T(ctl.E[c] / g.invN), T(0)};
             }
         } else {
-            cx<T> ev[6];
+            if constexpr (!E6) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) ev[c] = E6 ? eo[c] : v[6 + c];
+                for (int c = 0; c < 6; ++c) ev[c] = v[6 + c];
+            }
             const int fz = Elastic::fmap(kz, g.Nz), fy = Elastic::fmap(y, g.Ny), fxx = Elastic::fmap(kx, g.Nx);
             T xi[3] = {T(fz), T(fy), T(fxx)};
             cx<T> sg[6], o[6];
