@@ -31,7 +31,7 @@ struct Comp6 {
 //     lines -> sigma^ in S12 with 6 components per kz.  eps^ itself is not
 //     transformed again: the x pass keeps the spectrum of the strain it
 //     produced (the forward transform of the c2r of a Hermitian spectrum is
-//     that spectrum), so FFT_epsilon (:772) costs nothing here.
+//     that spectrum), so FFT_epsilon (:813) costs nothing here.
 //   FINAL: the ITER pass of the increment's last iterate without the r2c,
 //     storing eps, sigma and the return-map state: the FP64 ITER pass keeps
 //     them out of HBM (only that iterate needs them, update_old_Variable

@@ -601,19 +601,21 @@ __global__ void __launch_bounds__(256) k_bsums(Geo g, BPtr<T> p, double* out, in
 template <typename T, int BEH>
 __global__ void k_bupdate(Geo g, BPtr<T> p, int nsolve)
 {
+    // element-wise over the 6N tensors and the N scalars (consecutive
+    // threads, consecutive values)
+    const long long st = (long long)gridDim.x * blockDim.x;
+    const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (int s = 0; s < nsolve; ++s) {
-        for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < g.N; v += (long long)gridDim.x * blockDim.x) {
-            const long long i6 = (long long)s * p.sU + v * 6, i1 = (long long)s * p.sN + v;
-            if (BEH == BEH_EP) {
-                if (p.phi[i1] > T(0)) p.ep[i1] += p.dep[i1];
-                for (int c = 0; c < 6; ++c) p.sig_old[i6 + c] = p.sig[i6 + c];
-            }
-            for (int c = 0; c < 6; ++c) {
-                const T tmp = p.eps_old[i6 + c];
-                p.eps_old[i6 + c] = p.eps[i6 + c];
-                p.eps[i6 + c] += p.eps[i6 + c] - tmp;
-            }
+        const long long b6 = (long long)s * p.sU, b1 = (long long)s * p.sN;
+        for (long long i = i0; i < 6 * g.N; i += st) {
+            if (BEH == BEH_EP) p.sig_old[b6 + i] = p.sig[b6 + i];
+            const T e = p.eps[b6 + i], tmp = p.eps_old[b6 + i];
+            p.eps_old[b6 + i] = e;
+            p.eps[b6 + i] = e + (e - tmp);
         }
+        if (BEH == BEH_EP)
+            for (long long v = i0; v < g.N; v += st)
+                if (p.phi[b1 + v] > T(0)) p.ep[b1 + v] += p.dep[b1 + v];
     }
 }
 

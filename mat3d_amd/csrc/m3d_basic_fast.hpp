@@ -17,8 +17,14 @@ namespace bfast {
 #ifndef M3D_BZ_TILE
 #define M3D_BZ_TILE 2304
 #endif
+#ifndef M3D_BX_THREADS
+#define M3D_BX_THREADS 512
+#endif
+#ifndef M3D_BX_TILE
+#define M3D_BX_TILE 6144
+#endif
 constexpr int ZNT = M3D_BZ_THREADS; // threads of the z pass
-constexpr int XNT = 256;            // threads of the x pass
+constexpr int XNT = M3D_BX_THREADS; // threads of the x pass
 
 // z tile: TY lines (y) of 12 components each, TY*12*LZ <= M3D_BZ_TILE complex
 constexpr int zlines(int LZ, int Ny)
@@ -27,10 +33,10 @@ constexpr int zlines(int LZ, int Ny)
     while (ty < 8 && (ty * 2) * 12 * LZ <= M3D_BZ_TILE && Ny % (ty * 2) == 0) ty *= 2;
     return ty;
 }
-// x tile: TK kz values of 12 components
+// x tile: TK kz values of 12 components (TK * L <= M3D_BX_TILE / 12)
 constexpr int xtile(int L)
 {
-    const int tk = 3072 / (12 * L);
+    const int tk = M3D_BX_TILE / (12 * L);
     return tk < 1 ? 1 : tk;
 }
 // y tiles (the generic fast y pass with 12 / 6 components)

@@ -446,9 +446,9 @@ struct BasicRunner {
             }
             // update_old_Variable (:693-719); E += dE does not feed back (updateE recomputes E)
             if (beh == BEH_EP)
-                hipLaunchKernelGGL((k_bupdate<double, BEH_EP>), dim3(1024), dim3(256), 0, st, g12, bp, 1);
+                hipLaunchKernelGGL((k_bupdate<double, BEH_EP>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
             else
-                hipLaunchKernelGGL((k_bupdate<double, BEH_ED>), dim3(1024), dim3(256), 0, st, g12, bp, 1);
+                hipLaunchKernelGGL((k_bupdate<double, BEH_ED>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
         }
         HCHK(hipEventRecord(e.ev1, st));
         HCHK(hipStreamSynchronize(st));
