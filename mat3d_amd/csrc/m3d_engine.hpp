@@ -792,8 +792,11 @@ struct Engine {
         ypass<T, 1, BUF_MAIN>(g, p);
     }
 
+    // green = false: the iteration's update and stop check only, without the
+    // transforms that prepare the next iteration's Green output (the FP64
+    // anchor iteration of the polish, which the FP32 corrections follow)
     template <typename T>
-    void launch_iter(const Geo& g, const Ptr<T>& p)
+    void launch_iter(const Geo& g, const Ptr<T>& p, bool green = true)
     {
         zpass<T, MODE_IUF>(g, p);
         hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
@@ -804,6 +807,7 @@ struct Engine {
         ypass<T, -1, BUF_CHK_PREV>(g, p);
         xcheck<T, BUF_CHK_PREV>(g, p);
         hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.pchk);
+        if (!green) return;
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
         ypass<T, 1, BUF_MAIN>(g, p);
@@ -1097,10 +1101,11 @@ struct Engine {
                 // one FP64 iteration from the switch iterate A gives r = T(A) - A;
                 // the rest of the polish iterates the FP32 correction e <- r + M e
                 // (the load term drops out of M) with every stop-rule quantity
-                // taken in FP64 from A + e
+                // taken in FP64 from A + e.  The corrections transform e only,
+                // so that iteration stops after its update and stop check.
                 Ptr<double> pd = ptrs<double>(gd);
                 launch_start<double>(gd, pd);
-                launch_iter<double>(gd, pd);
+                launch_iter<double>(gd, pd, false);
                 HCHK(hipGetLastError());
                 HCHK(hipMemcpyAsync(hctl.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost, st));
                 HCHK(hipStreamSynchronize(st));
