@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     constexpr int C = START ? 12 : 6; // LDS columns per line
     constexpr int EO = START ? 6 : 0; // column of eps_0 within a line
     constexpr int W = TY * C;
+    constexpr int WS = W + M3D_BZ_PAD; // LDS row stride (complex values)
     constexpr int NZ = 2 * LZ;
     constexpr int NZH = LZ + 1;
     constexpr int NT = ZNT;
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[NBSUM][NW];
     cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* tw = sl + NZH * W;
+    cx<T>* tw = sl + NZH * WS;
 
     const int per = g.Nx * (g.Ny / TY);
     const int t = fast::xcd_remap(blockIdx.x, gridDim.x);
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
             const int yl = e / (NZH * 6);
             const int r = e - yl * (NZH * 6);
             const int kz = r / 6, c = r - kz * 6;
-            sl[kz * W + yl * 6 + c] = src[e];
+            sl[kz * WS + yl * 6 + c] = src[e];
         }
         __syncthreads();
         // c2r pre-processing of the eps lines; kz = 0 and Nz/2 keep real parts
@@ -107,23 +108,23 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         for (int e = tid; e < W * half; e += NT) {
             const int l = e % W, k = e / W;
             if (k == 0) {
-                const T x0 = sl[l].re, xl = sl[LZ * W + l].re;
+                const T x0 = sl[l].re, xl = sl[LZ * WS + l].re;
                 sl[l] = cx<T>{x0 + xl, x0 - xl};
             } else {
                 const int kk = LZ - k;
-                const cx<T> Xk = sl[k * W + l], Xc = sl[kk * W + l];
+                const cx<T> Xk = sl[k * WS + l], Xc = sl[kk * WS + l];
                 cx<T> w = tw[k];
                 w.im = -w.im;
-                sl[k * W + l] = (Xk + cconj(Xc)) + rot90<1>(cmul(w, Xk - cconj(Xc)));
+                sl[k * WS + l] = (Xk + cconj(Xc)) + rot90<1>(cmul(w, Xk - cconj(Xc)));
                 if (kk != k) {
                     cx<T> w2 = tw[kk];
                     w2.im = -w2.im;
-                    sl[kk * W + l] = (Xc + cconj(Xk)) + rot90<1>(cmul(w2, Xc - cconj(Xk)));
+                    sl[kk * WS + l] = (Xc + cconj(Xk)) + rot90<1>(cmul(w2, Xc - cconj(Xk)));
                 }
             }
         }
         __syncthreads();
-        fast::fft<T, LZ, W, NT, 1, 2>(sl, tw);
+        fast::fft<T, LZ, W, NT, 1, 2, 0, 1, WS>(sl, tw);
     } else {
         __syncthreads();
     }
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         if constexpr (!START) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-                const cx<T> u = sl[m * W + yl * C + c];
+                const cx<T> u = sl[m * WS + yl * C + c];
                 e2[c] = u.re;
                 e2[6 + c] = u.im;
             }
@@ -195,8 +196,8 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         // sigma over the slots this thread read eps from (ITER), or beside eps (START)
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            sl[m * W + yl * C + c] = cx<T>{s2[c], s2[6 + c]};
-            if (START) sl[m * W + yl * C + EO + c] = cx<T>{e2[c], e2[6 + c]};
+            sl[m * WS + yl * C + c] = cx<T>{s2[c], s2[6 + c]};
+            if (START) sl[m * WS + yl * C + EO + c] = cx<T>{e2[c], e2[6 + c]};
         }
     }
     if (bad) ctl.err = 1;
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
     __syncthreads();
 
     // r2c of the W lines
-    fast::fft<T, LZ, W, NT, -1, 2>(sl, tw);
+    fast::fft<T, LZ, W, NT, -1, 2, 0, 1, WS>(sl, tw);
     {
         constexpr int half = LZ / 2 + 1;
         for (int e = tid; e < W * half; e += NT) {
@@ -228,14 +229,14 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
             if (k == 0) {
                 const cx<T> z0 = sl[l];
                 sl[l] = cx<T>{z0.re + z0.im, T(0)};
-                sl[LZ * W + l] = cx<T>{z0.re - z0.im, T(0)};
+                sl[LZ * WS + l] = cx<T>{z0.re - z0.im, T(0)};
             } else {
                 const int kk = LZ - k;
-                const cx<T> zk = sl[k * W + l], zc = cconj(sl[kk * W + l]);
+                const cx<T> zk = sl[k * WS + l], zc = cconj(sl[kk * WS + l]);
                 const cx<T> E = cscale(zk + zc, T(0.5));
                 const cx<T> O = cscale(rot90<-1>(zk - zc), T(0.5));
-                sl[k * W + l] = E + cmul(tw[k], O);
-                if (kk != k) sl[kk * W + l] = cconj(E) + cmul(tw[kk], cconj(O));
+                sl[k * WS + l] = E + cmul(tw[k], O);
+                if (kk != k) sl[kk * WS + l] = cconj(E) + cmul(tw[kk], cconj(O));
             }
         }
     }
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
         const int yl = e / (NZH * C);
         const int r = e - yl * (NZH * C);
         const int kz = r / C, c = r - kz * C;
-        dst[e] = sl[kz * W + yl * C + c];
+        dst[e] = sl[kz * WS + yl * C + c];
     }
 }
 
@@ -389,7 +390,7 @@ bool z_at(const Geo& g, const BPtr<T>& p, hipStream_t st)
     constexpr int TY = zlines(LZ, 2 * LZ);
     if (g.TY != TY) throw std::runtime_error("basic z tile mismatch");
     constexpr int C = MODE == BMODE_START ? 12 : 6;
-    const size_t sm = ((size_t)(LZ + 1) * TY * C + 2 * LZ) * sizeof(cx<T>);
+    const size_t sm = ((size_t)(LZ + 1) * (TY * C + M3D_BZ_PAD) + 2 * LZ) * sizeof(cx<T>);
     hipLaunchKernelGGL((kzb<T, BEH, MODE, LZ, TY, LIN>), dim3(g.nbz), dim3(ZNT), sm, st, g, p);
     return true;
 }

@@ -17,6 +17,9 @@ namespace bfast {
 #ifndef M3D_BZ_TILE
 #define M3D_BZ_TILE 2304
 #endif
+#ifndef M3D_BZ_PAD
+#define M3D_BZ_PAD 0 // complex values of padding per LDS row of the z pass
+#endif
 #ifndef M3D_BX_THREADS
 #define M3D_BX_THREADS 512
 #endif
