@@ -412,12 +412,61 @@ struct RowTile {
     static constexpr int PER = (N + NT - 1) / NT;
 };
 
+// a * b for a, b < 2^24 as one full-rate v_mul_u32_u24 (low 32 bits of the
+// product).  Written out because the compiler selects the quarter-rate
+// 32-bit multiply when the operand that bounds b sits in another basic block.
+__device__ __forceinline__ unsigned mul_u24(unsigned a, unsigned b)
+{
+    unsigned r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// plain rows whose byte stride s satisfies s < 2^24 and ROWS * s < 2^32: a
+// row's byte offset is one full-rate 24-bit multiply from the tile's
+// (uniform) base, and no 64-bit address arithmetic is done per access
+template <typename T, int ROWS>
+__device__ __forceinline__ bool rows_fit32(const RowMap& rm)
+{
+    constexpr long long lim = (1ll << 24) < (1ll << 32) / ROWS ? (1ll << 24) : (1ll << 32) / ROWS;
+    return rm.sh == 0 && rm.mask == 0 && rm.a >= 0 && rm.a * (long long)sizeof(cx<T>) < lim;
+}
+
 template <typename T, int ROWS, int W, int NT, bool VEC, typename V, int PER>
 __device__ __forceinline__ void rows_load(V (&r)[PER], const cx<T>* __restrict__ base, RowMap rm, int wv)
 {
     using RT = RowTile<T, ROWS, W, NT, VEC>;
     static_assert(PER == RT::PER, "register tile size");
     const int t0 = lane_tid();
+    if (rows_fit32<T, ROWS>(rm)) {
+        const unsigned sb = (unsigned)(rm.a * (long long)sizeof(cx<T>)); // < 2^24
+        const char* bb = reinterpret_cast<const char*>(base);
+        auto addr = [&](int e, int& jj) {
+            const int row = cdiv<RT::WG, RT::PER * NT>(e);
+            jj = e - row * RT::WG;
+            return reinterpret_cast<const V*>(bb + (mul_u24((unsigned)row, sb) + (unsigned)(jj * (int)sizeof(V))));
+        };
+        if (wv == W) {
+            // every column valid (all kz tiles but a ragged last one)
+#pragma unroll
+            for (int i = 0; i < RT::PER; ++i) {
+                const int e = t0 + i * NT;
+                int jj;
+                if (RT::N % NT == 0 || e < RT::N) r[i] = *addr(e, jj);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < RT::PER; ++i) {
+                const int e = t0 + i * NT;
+                if (RT::N % NT == 0 || e < RT::N) {
+                    int jj;
+                    const V* a = addr(e, jj);
+                    r[i] = jj * RT::G < wv ? *a : V(0);
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < RT::PER; ++i) {
         const int e = t0 + i * NT;
@@ -461,6 +510,22 @@ __device__ __forceinline__ void lds_to_rows(const cx<T>* __restrict__ sl, cx<T>*
     constexpr int N = ROWS * WG;
     constexpr int PER = (N + NT - 1) / NT;
     const int t0 = lane_tid();
+    if (rows_fit32<T, ROWS>(rm)) {
+        const unsigned sb = (unsigned)(rm.a * (long long)sizeof(cx<T>)); // < 2^24
+        char* bb = reinterpret_cast<char*>(base);
+        const bool full = wv == W;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = t0 + i * NT;
+            if (N % NT == 0 || e < N) {
+                const int row = cdiv<WG, PER * NT>(e), jj = e - row * WG;
+                if (full || jj * G < wv)
+                    *reinterpret_cast<V*>(bb + (mul_u24((unsigned)row, sb) + (unsigned)(jj * (int)sizeof(V)))) =
+                        *reinterpret_cast<const V*>(sl + row * W + jj * G);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int e = t0 + i * NT;
@@ -914,9 +979,15 @@ __device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* 
     RowMap rm;
     cx<T>* base = const_cast<cx<T>*>(xy_rows<T, C, KIND, WHICH>(g, p, s, o, kz0, rm));
     const cx<T>* twg = KIND == 0 ? p.twy : p.twx;
-    for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
     constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
-    rows_to_lds<T, L, W, NT, VEC>(sl, base, rm, wv);
+    {
+        // the tile's loads are issued before the twiddle table is copied to LDS
+        using RT = RowTile<T, L, W, NT, VEC>;
+        typename RT::V r[RT::PER];
+        rows_load<T, L, W, NT, VEC>(r, base, rm, wv);
+        for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+        rows_commit<T, L, W, NT, VEC>(sl, r);
+    }
     __syncthreads();
     // the Green step's reference medium (KIND 1 only: other users leave rve_of unset)
     const T* prm = nullptr;
