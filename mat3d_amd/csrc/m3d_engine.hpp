@@ -646,8 +646,11 @@ struct Engine {
         }
         const size_t zsm = (size_t)g.TY * C * g.lsz * sizeof(cx<T>);
         const dim3 grid(g.nsolve * g.nbz);
-        // the generic kernel adds r of anchored solves at run time
-        constexpr int GM = MODE == MODE_IUFR ? MODE_IUF : MODE;
+        // the generic kernel adds r of anchored solves at run time; the anchor
+        // mode is the specialised kernels' only (callers check g.fz)
+        static_assert(MODE != MODE_IUA || sizeof(T) == 8, "anchor mode is FP64");
+        if (MODE == MODE_IUA) throw ArgError{M3D_ERR_STATE, "anchor z pass without the specialised kernels"};
+        constexpr int GM = (MODE == MODE_IUFR || MODE == MODE_IUA) ? MODE_IUF : MODE;
         if (g.gen) hipLaunchKernelGGL((k_zpass<T, PH, GM, true>), grid, dim3(ZTHREADS), zsm, st, g, p);
         else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
@@ -798,7 +801,14 @@ struct Engine {
     template <typename T>
     void launch_iter(const Geo& g, const Ptr<T>& p, bool green = true)
     {
-        zpass<T, MODE_IUF>(g, p);
+        // the FP64 anchor iteration on the specialised kernels also writes r
+        // and e (MODE_IUA; k_anchor otherwise)
+        if constexpr (sizeof(T) == 8) {
+            if (!green && g.fz) zpass<T, MODE_IUA>(g, p);
+            else zpass<T, MODE_IUF>(g, p);
+        } else {
+            zpass<T, MODE_IUF>(g, p);
+        }
         hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
         zpass<T, MODE_CHK_CUR>(g, p);
         ypass<T, -1, BUF_CHK_CUR>(g, p);
@@ -894,13 +904,14 @@ struct Engine {
         run_graph(chunk_for(g), [&]() { launch_iter<T>(g, p); }, hctl);
     }
 
-    // anchored polish of the solves flagged corr (see k_anchor)
+    // anchored polish of the solves flagged corr (r and e from k_anchor, or
+    // from the anchor iteration's z pass on the specialised kernels)
     void run_corr(Geo gf, Geo gd, std::vector<Ctl>& hctl)
     {
         gf.nsolve = gd.nsolve = nsolve;
         Ptr<float> pf = ptrs<float>(gf);
         Ptr<double> pd = ptrs<double>(gd);
-        hipLaunchKernelGGL((k_anchor<C>), dim3(1024), dim3(256), 0, st, gd, pd, pf, ctl);
+        if (!gd.fz) hipLaunchKernelGGL((k_anchor<C>), dim3(1024), dim3(256), 0, st, gd, pd, pf, ctl);
         launch_start_corr(gf, pf);
         HCHK(hipGetLastError());
         run_graph(chunk_for(gf), [&]() { launch_iter_corr(gf, pf, gd, pd); }, hctl);

@@ -624,7 +624,8 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
     const int t = b - s * per;
     const int x = t / (g.Ny / TY);
     const int y0 = (t - x * (g.Ny / TY)) * TY;
-    constexpr bool IUF = MODE == MODE_IUF || MODE == MODE_IUFR;
+    constexpr bool IUF = MODE == MODE_IUF || MODE == MODE_IUFR || MODE == MODE_IUA;
+    constexpr bool ANC = MODE == MODE_IUA; // anchor iteration: no forward transform
     constexpr bool CHK = MODE == MODE_CHK_CUR || MODE == MODE_CHK_PREV;
     const Ctl& ctl = p.ctl[s];
     if (ctl.done) return;
@@ -654,6 +655,13 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
         }
     }
     const float* Rin = MODE == MODE_IUFR ? p.R + (long long)s * p.sU + vox0 * C : nullptr;
+    // anchor iteration: r and the correction e of iterate it + 1 (FP32 ring)
+    float* Ranc = nullptr;
+    float* Eanc = nullptr;
+    if constexpr (ANC) {
+        Ranc = const_cast<float*>(p.R) + (long long)s * p.sU + vox0 * C;
+        Eanc = const_cast<float*>(e_buf(p, it + 1, s)) + vox0 * C;
+    }
     Uin += vox0 * C;
     if (Uout) Uout += vox0 * C;
 
@@ -754,14 +762,24 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
                     un[C + c] = u1[c];
                 }
                 vstore<T, 2 * C>(Uout + (long long)v0 * C, un);
+                if constexpr (ANC) {
+                    // r = T(A) - A in FP64, kept in FP32 (the difference of the stored iterates)
+                    float d[2 * C];
+#pragma unroll
+                    for (int c = 0; c < 2 * C; ++c) d[c] = (float)(un[c] - u[i][c]);
+                    vstore<float, 2 * C>(Ranc + (long long)v0 * C, d);
+                    vstore<float, 2 * C>(Eanc + (long long)v0 * C, d);
+                }
             }
             if (MODE == MODE_FWD || IUF) {
                 PH::dual(k0, u0, o0);
                 PH::dual(k1, u1, o1);
 #pragma unroll
                 for (int c = 0; c < C; ++c) acc += (double)o0[c] + (double)o1[c];
-                PH::polar(k0, u0, o0);
-                PH::polar(k1, u1, o1);
+                if constexpr (!ANC) {
+                    PH::polar(k0, u0, o0);
+                    PH::polar(k1, u1, o1);
+                }
             } else if (ctl.sb) {
                 PH::dual(k0, u0, o0);
                 PH::dual(k1, u1, o1);
@@ -769,14 +787,17 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
 #pragma unroll
                 for (int c = 0; c < C; ++c) { o0[c] = u0[c]; o1[c] = u1[c]; }
             }
+            if constexpr (!ANC) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
+                for (int c = 0; c < C; ++c) slot[c] = cx<T>{o0[c], o1[c]};
+            }
         }
     }
     if (MODE == MODE_FWD || IUF) {
         const double bs = block_sum(acc, red);
         if (tid == 0) p.psum[(long long)s * g.nbz + t] = bs;
     }
+    if constexpr (ANC) return;
     __syncthreads();
 
     fft<T, LZ, W, NT, -1, 2>(sl, tw);

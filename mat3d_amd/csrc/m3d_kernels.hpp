@@ -37,8 +37,11 @@ constexpr int XYTHREADS = 512;
 constexpr int EPT = 16;  // complex values per thread per pass
 
 // z-pass modes: forward only, update (inverse, update, forward), check FFT of
-// the current / previous iterate, update of an anchored FP32 correction (adds r)
-enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3, MODE_IUFR = 4 };
+// the current / previous iterate, update of an anchored FP32 correction (adds
+// r), and the FP64 anchor iteration of the polish (inverse and update only;
+// writes r = u_new - u_old and the first correction e = r in FP32, the
+// specialised z pass only)
+enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3, MODE_IUFR = 4, MODE_IUA = 5 };
 enum { BUF_MAIN = 0, BUF_CHK_CUR = 2, BUF_CHK_PREV = 3, BUF_K = 4 };
 
 // per-solve control block, read and written on the device only
