@@ -648,9 +648,10 @@ struct Engine {
         const dim3 grid(g.nsolve * g.nbz);
         // the generic kernel adds r of anchored solves at run time; the anchor
         // mode is the specialised kernels' only (callers check g.fz)
-        static_assert(MODE != MODE_IUA || sizeof(T) == 8, "anchor mode is FP64");
-        if (MODE == MODE_IUA) throw ArgError{M3D_ERR_STATE, "anchor z pass without the specialised kernels"};
-        constexpr int GM = (MODE == MODE_IUFR || MODE == MODE_IUA) ? MODE_IUF : MODE;
+        static_assert((MODE != MODE_IUA && MODE != MODE_FWDP) || sizeof(T) == 8, "polish modes are FP64");
+        if (MODE == MODE_IUA || MODE == MODE_FWDP)
+            throw ArgError{M3D_ERR_STATE, "polish z pass without the specialised kernels"};
+        constexpr int GM = (MODE == MODE_IUFR || MODE == MODE_IUA) ? MODE_IUF : (MODE == MODE_FWDP ? MODE_FWD : MODE);
         if (g.gen) hipLaunchKernelGGL((k_zpass<T, PH, GM, true>), grid, dim3(ZTHREADS), zsm, st, g, p);
         else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
@@ -786,9 +787,16 @@ struct Engine {
     }
 
     template <typename T>
-    void launch_start(const Geo& g, const Ptr<T>& p)
+    // promote: the FP64 start of the polish reads the FP32 iterate and writes
+    // its FP64 copy itself (specialised kernels; k_promote otherwise)
+    void launch_start(const Geo& g, const Ptr<T>& p, bool promote = false)
     {
-        zpass<T, MODE_FWD>(g, p);
+        if constexpr (sizeof(T) == 8) {
+            if (promote) zpass<T, MODE_FWDP>(g, p);
+            else zpass<T, MODE_FWD>(g, p);
+        } else {
+            zpass<T, MODE_FWD>(g, p);
+        }
         hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
@@ -1105,7 +1113,11 @@ struct Engine {
                 hctl[s].tol_eq = tol_eq;
             }
             HCHK(hipMemcpyAsync(ctl, hctl.data(), sizeof(Ctl) * nsolve, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, ptrs<float>(gf), ptrs<double>(gd), ctl);
+            // promote the FP32 iterates to FP64: inside the polish's first z pass
+            // on the specialised kernels, else by k_promote
+            const bool fused_promote = refine && gd.fz;
+            if (!fused_promote)
+                hipLaunchKernelGGL((k_promote<C>), dim3(1024), dim3(256), 0, st, gd, ptrs<float>(gf), ptrs<double>(gd), ctl);
             bool any = false;
             for (int s = 0; s < nsolve; ++s) any = any || (hctl[s].done == 0);
             if (any && refine) {
@@ -1115,7 +1127,7 @@ struct Engine {
                 // taken in FP64 from A + e.  The corrections transform e only,
                 // so that iteration stops after its update and stop check.
                 Ptr<double> pd = ptrs<double>(gd);
-                launch_start<double>(gd, pd);
+                launch_start<double>(gd, pd, fused_promote);
                 launch_iter<double>(gd, pd, false);
                 HCHK(hipGetLastError());
                 HCHK(hipMemcpyAsync(hctl.data(), ctl, sizeof(Ctl) * nsolve, hipMemcpyDeviceToHost, st));

@@ -641,9 +641,17 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
     const int rv = p.rve_of[s];
     const uint8_t* ph = p.ph + (long long)rv * p.sPh + vox0;
     const T* coef = p.coef + (long long)rv * p.sCoef;
+    constexpr bool FWD = MODE == MODE_FWD || MODE == MODE_FWDP;
     const T* Uin;
     T* Uout = nullptr;
-    if (MODE == MODE_FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
+    // FWDP: the FP32 iterate `it` (FP32 ring), promoted to the FP64 iterate `it`
+    const float* Uin32 = nullptr;
+    T* Uprom = nullptr;
+    if constexpr (MODE == MODE_FWDP) {
+        Uin32 = e_buf(p, it, s) + vox0 * C;
+        Uprom = u_buf(p, it, s) + vox0 * C;
+    }
+    if (FWD || MODE == MODE_CHK_CUR) Uin = u_buf(p, it, s);
     else if (IUF) { Uin = u_buf(p, it, s); Uout = u_buf(p, it + 1, s); }
     else Uin = u_buf(p, it - 1, s);
     // anchored polish: the FP64 check reads anchor + FP32 correction
@@ -682,7 +690,15 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
         if (NPAIR % NT == 0 || q < NPAIR) {
             const int yl = q / LZ, m = q - yl * LZ;
             const int v0 = yl * NZ + 2 * m;
-            vload<T, 2 * C>(Uin + (long long)v0 * C, u[i]);
+            if constexpr (MODE == MODE_FWDP) {
+                float f[2 * C];
+                vload<float, 2 * C>(Uin32 + (long long)v0 * C, f);
+#pragma unroll
+                for (int c = 0; c < 2 * C; ++c) u[i][c] = (T)f[c];
+                vstore<T, 2 * C>(Uprom + (long long)v0 * C, u[i]);
+            } else {
+                vload<T, 2 * C>(Uin + (long long)v0 * C, u[i]);
+            }
             if constexpr (MODE == MODE_IUFR) vload<T, 2 * C>(Rin + (long long)v0 * C, rr[i]);
             pp[i] = *reinterpret_cast<const uint16_t*>(ph + v0);
             if constexpr (sizeof(T) == 8 && CHK) {
@@ -771,7 +787,7 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
                     vstore<float, 2 * C>(Eanc + (long long)v0 * C, d);
                 }
             }
-            if (MODE == MODE_FWD || IUF) {
+            if (FWD || IUF) {
                 PH::dual(k0, u0, o0);
                 PH::dual(k1, u1, o1);
 #pragma unroll
@@ -793,7 +809,7 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
             }
         }
     }
-    if (MODE == MODE_FWD || IUF) {
+    if (FWD || IUF) {
         const double bs = block_sum(acc, red);
         if (tid == 0) p.psum[(long long)s * g.nbz + t] = bs;
     }
@@ -821,7 +837,7 @@ __device__ __forceinline__ void kz_tile(const Geo& g, const Ptr<T>& p, int b, cx
     }
     __syncthreads();
     {
-        cx<T>* dst = ((MODE == MODE_FWD || IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
+        cx<T>* dst = ((FWD || IUF) ? p.S : p.K) + (long long)s * p.sS + spec0;
         lds_to_zchunk<T, TY, NZH, C, NT>(sl, dst, RS);
     }
 }

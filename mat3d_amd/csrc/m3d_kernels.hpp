@@ -38,10 +38,12 @@ constexpr int EPT = 16;  // complex values per thread per pass
 
 // z-pass modes: forward only, update (inverse, update, forward), check FFT of
 // the current / previous iterate, update of an anchored FP32 correction (adds
-// r), and the FP64 anchor iteration of the polish (inverse and update only;
-// writes r = u_new - u_old and the first correction e = r in FP32, the
-// specialised z pass only)
-enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3, MODE_IUFR = 4, MODE_IUA = 5 };
+// r), the FP64 anchor iteration of the polish (inverse and update only;
+// writes r = u_new - u_old and the first correction e = r in FP32), and the
+// FP64 start of the polish from the FP32 iterate (forward only; reads the
+// FP32 ring, writes the promoted FP64 iterate).  The last two: the
+// specialised z pass only.
+enum { MODE_FWD = 0, MODE_IUF = 1, MODE_CHK_CUR = 2, MODE_CHK_PREV = 3, MODE_IUFR = 4, MODE_IUA = 5, MODE_FWDP = 6 };
 enum { BUF_MAIN = 0, BUF_CHK_CUR = 2, BUF_CHK_PREV = 3, BUF_K = 4 };
 
 // per-solve control block, read and written on the device only
