@@ -44,10 +44,10 @@ constexpr size_t LDS_MAX = 65536;
 #define M3D_XPAD 128
 #endif
 
-// byte alignment of the spectrum rows of the specialised kernels (0: rows
-// packed at Nzh*C complex values).  64-byte rows were measured slower in an
-// A/B on one box at 256^3 (z pass +6%, y passes +7%, x pass with the Green
-// step -1%): the default keeps the rows packed.
+// byte alignment of the spectrum rows of the specialised kernels (0: 16
+// bytes, whole kz slots of padding; rows of even C are then packed at Nzh*C
+// complex values).  64-byte rows were measured slower in an A/B on one box at
+// 256^3 (z pass +6%, y passes +7%, x pass with the Green step -1%).
 #ifndef M3D_ROW_ALIGN
 #define M3D_ROW_ALIGN 0
 #endif
@@ -201,13 +201,16 @@ Geo make_geo(int Nx, int Ny, int Nz, int C)
     // at (x * Ny + y) * Nzh * C: plain rows without padding)
     const long long rowc = (long long)g.Nzh * C;
     const bool all_fast = g.fz && g.fy && g.fx;
-    // the specialised kernels start every row on a 64-byte boundary (whole kz
-    // slots of padding): their y and x passes move a row's kz tile as one
-    // contiguous chunk, and a chunk that straddles 64-byte sectors is written
-    // as partial sectors (FP32, C = 6, 256^3: rows of 129 slots, 6192 bytes,
-    // padded to 132 slots)
+    // the specialised kernels may start every row on an M3D_ROW_ALIGN-byte
+    // boundary (whole kz slots of padding): their y and x passes move a row's
+    // kz tile as one contiguous chunk, and a chunk that straddles 64-byte
+    // sectors is written as partial sectors (measured: 64-byte rows made the
+    // z and y passes slower).  By default rows are padded to 16 bytes only,
+    // which changes nothing for even C and gives odd C in FP32 (thermal, rows
+    // of Nzh*3 values) an even row pitch, so that its y and x passes move
+    // 16-byte pairs too
     long long rowp = rowc;
-    constexpr long long row_align = M3D_ROW_ALIGN > 0 ? M3D_ROW_ALIGN : 1;
+    constexpr long long row_align = M3D_ROW_ALIGN > 0 ? M3D_ROW_ALIGN : 16;
     if (all_fast)
         while ((rowp * (long long)csz) % row_align) rowp += C;
     g.lxb = 0;

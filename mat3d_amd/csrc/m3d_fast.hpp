@@ -1016,21 +1016,44 @@ __device__ __forceinline__ void xy_body(const Geo& g, const Ptr<T>& p, const T* 
     RowMap rm;
     cx<T>* base = const_cast<cx<T>*>(xy_rows<T, C, KIND, WHICH>(g, p, s, o, kz0, rm));
     const cx<T>* twg = KIND == 0 ? p.twy : p.twx;
-    constexpr bool VEC = (sizeof(T) == 4) && (C % 2 == 0);
-    {
+    // FP32 tiles move pairs of complex values (16 bytes) per access: always for
+    // even C; for odd C (thermal) when the tile's row chunks are pairs of
+    // 16-byte-aligned values (even row pitch, even chunk width), a uniform
+    // test per tile, else single values
+    constexpr bool VEC_EVEN = (sizeof(T) == 4) && (C % 2 == 0);
+    constexpr bool VEC_ODD = (sizeof(T) == 4) && (C % 2 != 0) && ((TK * C) % 2 == 0);
+    bool vec = VEC_EVEN;
+    if constexpr (VEC_ODD)
+        vec = (reinterpret_cast<unsigned long long>(base) & 15) == 0 && rm.sh == 0 && rm.mask == 0 && (rm.a & 1) == 0 &&
+              (wv & 1) == 0;
+    auto load = [&](auto vc) {
+        constexpr bool VEC = decltype(vc)::value;
         // the tile's loads are issued before the twiddle table is copied to LDS
         using RT = RowTile<T, L, W, NT, VEC>;
         typename RT::V r[RT::PER];
         rows_load<T, L, W, NT, VEC>(r, base, rm, wv);
         for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
         rows_commit<T, L, W, NT, VEC>(sl, r);
+    };
+    if constexpr (VEC_ODD) {
+        if (vec) load(std::true_type{});
+        else load(std::false_type{});
+    } else {
+        load(std::integral_constant<bool, VEC_EVEN>{});
     }
     __syncthreads();
     // the Green step's reference medium (KIND 1 only: other users leave rve_of unset)
     const T* prm = nullptr;
     if constexpr (KIND == 1) prm = prm_all + 2 * p.rve_of[s];
     xy_tile<T, PH, KIND, DIR, WHICH, L, TK, NT, RP>(g, p, p.ctl[s], prm, sl, tw, red, s, o, kz0, nk, t);
-    if constexpr (KIND != 2) lds_to_rows<T, L, W, NT, VEC>(sl, base, rm, wv);
+    if constexpr (KIND != 2) {
+        if constexpr (VEC_ODD) {
+            if (vec) lds_to_rows<T, L, W, NT, true>(sl, base, rm, wv);
+            else lds_to_rows<T, L, W, NT, false>(sl, base, rm, wv);
+        } else {
+            lds_to_rows<T, L, W, NT, VEC_EVEN>(sl, base, rm, wv);
+        }
+    }
 }
 
 // one workgroup per tile; LOOP (the solver's check passes only): a capped
