@@ -44,7 +44,7 @@ struct Comp6 {
 // written by the start stage: the iterations read 48 bytes of history per
 // voxel instead of 96.
 template <typename T, int BEH, int MODE, int LZ, int TY, bool LIN = false>
-__global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
+__global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
 {
     constexpr bool FIN = MODE == BMODE_FINAL;
     constexpr bool START = MODE == BMODE_START;
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(ZNT) kzb(Geo g, BPtr<T> p)
 // Both write the new eps^ (unnormalised, before the inverse x FFT) to Eh,
 // one contiguous [kx][kzl][c] block per tile, for the next iteration.
 template <typename T, int L, int TK, bool E6>
-__global__ void __launch_bounds__(XNT) kxb(Geo g, BPtr<T> p)
+__global__ void __launch_bounds__(XNT) M3D_BX_ATTR kxb(Geo g, BPtr<T> p)
 {
     constexpr int C = E6 ? 6 : 12; // LDS columns per kz
     constexpr int W = TK * C;

@@ -26,6 +26,24 @@ namespace bfast {
 #ifndef M3D_BX_TILE
 #define M3D_BX_TILE 6144
 #endif
+// minimum waves per SIMD the register allocator must leave room for in the
+// z and x passes (0: no constraint)
+#ifndef M3D_BZ_WAVES
+#define M3D_BZ_WAVES 0
+#endif
+#ifndef M3D_BX_WAVES
+#define M3D_BX_WAVES 0
+#endif
+#if M3D_BZ_WAVES > 0
+#define M3D_BZ_ATTR __attribute__((amdgpu_waves_per_eu(M3D_BZ_WAVES)))
+#else
+#define M3D_BZ_ATTR
+#endif
+#if M3D_BX_WAVES > 0
+#define M3D_BX_ATTR __attribute__((amdgpu_waves_per_eu(M3D_BX_WAVES)))
+#else
+#define M3D_BX_ATTR
+#endif
 constexpr int ZNT = M3D_BZ_THREADS; // threads of the z pass
 constexpr int XNT = M3D_BX_THREADS; // threads of the x pass
 
