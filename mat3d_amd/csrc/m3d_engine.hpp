@@ -52,9 +52,10 @@ constexpr size_t LDS_MAX = 65536;
 #define M3D_ROW_ALIGN 0
 #endif
 
-// workgroups of a check-pass launch (the check kernels loop over their tiles)
+// workgroups of a check-pass launch (the check kernels loop over their
+// tiles; 0: the physics' own choice, PH::chk_grid)
 #ifndef M3D_CHK_WG
-#define M3D_CHK_WG 32768
+#define M3D_CHK_WG 0
 #endif
 
 struct HipError {
@@ -660,10 +661,11 @@ struct Engine {
     }
 
     // workgroups of a launch: one per tile, or for the check passes (which
-    // loop over their tiles) at most M3D_CHK_WG
+    // loop over their tiles) at most M3D_CHK_WG (PH::chk_grid by default)
     static unsigned grid_of(long long tiles, bool check)
     {
-        return (unsigned)(check ? std::min<long long>(tiles, M3D_CHK_WG) : tiles);
+        constexpr long long cap = M3D_CHK_WG > 0 ? M3D_CHK_WG : PH::chk_grid;
+        return (unsigned)(check ? std::min<long long>(tiles, cap) : tiles);
     }
 
     template <typename T, int KIND, int DIR, int WHICH, int L, int TK, int NTH, int RP = 0>
