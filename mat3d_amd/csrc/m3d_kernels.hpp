@@ -267,18 +267,23 @@ __device__ __forceinline__ void green_elastic_k(const T xi[3], const cx<T> t[6],
             o[q] = r - cscale(s, b * xi[i] * xi[j]);
         }
     } else {
-        // n n = xi xi / n2 ; s/n2 = n.t.n ; w/|xi| = t n
-        cx<T> trt = t[0] + t[1] + t[2];
-        cx<T> sn = cscale(s, inv);                // n.t.n
-        cx<T> trPt = trt - sn;                    // tr(P t)
+        // n n = xi xi / n2 ; s/n2 = n.t.n ; w/|xi| = t n ; with
+        // (P t P)_ij = t_ij - n_i w'_j - w'_i n_j + sn n_i n_j (w' = t n) and
+        // P_ij = delta_ij - xi_i xi_j / n2 the output collects into
+        //   o_ij = 2mu0 t_ij - (2mu0/n2)(xi_i w_j + xi_j w_i) + xi_i xi_j G + delta_ij D
+        // with D = c3 tr(P t) and G = (2mu0 sn - D) / n2, formed once per frequency
+        const cx<T> trt = t[0] + t[1] + t[2];
+        const cx<T> sn = cscale(s, inv); // n.t.n
+        const cx<T> D = cscale(trt - sn, K.c3);
+        const cx<T> G = cscale(cscale(sn, K.m2) - D, inv);
+        const T a = K.m2 * inv;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int i = pi[q], j = pj[q];
-            const T nn = xi[i] * xi[j] * inv;
-            const T Pij = (i == j ? T(1) : T(0)) - nn;
-            // (P t P)_ij = t_ij - n_i w'_j - w'_i n_j + sn n_i n_j, w' = t n
-            cx<T> ptp = t[q] - cscale(cscale(w[j], xi[i]) + cscale(w[i], xi[j]), inv) + cscale(sn, nn);
-            o[q] = cscale(trPt, K.c3 * Pij) + cscale(ptp, K.m2);
+            const cx<T> x = cscale(w[j], xi[i]) + cscale(w[i], xi[j]);
+            cx<T> r = cscale(t[q], K.m2) - cscale(x, a) + cscale(G, xi[i] * xi[j]);
+            if (i == j) r = r + D;
+            o[q] = r;
         }
     }
 }
