@@ -41,11 +41,11 @@ METRIC = "voxel-iters/s at 256³ (FP32+FP64 polish vs FP64 GPU), C_eff rel err"
 # (profiles/elastic256_fibres_hbm_traffic.md)
 MEASURED_TRAFFIC_256 = {
     "zpass_update": 9.805e9,
-    "ypass_fwd": 5.104e9,
-    "xpass_green": 5.150e9,
-    "ypass_inv": 5.087e9,
+    "ypass_fwd": 5.128e9,
+    "xpass_green": 4.999e9,
+    "ypass_inv": 5.086e9,
     "zpass_check": 4.952e9,
-    "xpass_check": 2.497e9,
+    "xpass_check": 2.797e9,
 }
 
 
