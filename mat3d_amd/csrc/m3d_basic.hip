@@ -39,19 +39,23 @@ struct Comp6 {
 //     in the S6 buffer it was read from (the x pass writes the other one).
 // All modes: FP64 partial sums of sigma and eps (updateE :583-594).  The
 // damage variable is written every iteration: the next one compares with it.
-// The J2 trial stress sigma_old + C (eps - eps_old) is formed, from the
-// start stage on, as sigma_star + C eps with sigma_star = sigma_old - C eps_old
-// written by the start stage: the iterations read 48 bytes of history per
-// voxel instead of 96.
+// The J2 trial stress sigma_old + C (eps - eps_old) is formed as
+// sigma_star + C eps with sigma_star = sigma_old - C eps_old (FP64): every
+// stage reads 48 bytes of history per voxel instead of 96.  FINAL (FP64 "ep")
+// also ends the increment: it writes the export sums (export_homogen_var
+// :722-746) and applies update_old_Variable (:693-719) -- eps_p += d_eps_p
+// where phi > 0, sigma_old = sigma, eps_old = eps, eps += eps - eps_old --
+// and the next sigma_star, so no other pass reads the fields for them.
 template <typename T, int BEH, int MODE, int LZ, int TY, bool LIN = false>
 __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
 {
     constexpr bool FIN = MODE == BMODE_FINAL;
     constexpr bool START = MODE == BMODE_START;
     constexpr bool STORE = !(MODE == BMODE_ITER && sizeof(T) == 8);
-    // FP64 "ep": the start stage writes sigma_star = sigma_old - C eps_old and
-    // the iterations read it in place of sigma_old and eps_old
+    // FP64 "ep": every stage reads sigma_star = sigma_old - C eps_old in place
+    // of sigma_old and eps_old; FINAL ends the increment (FUSE)
     constexpr bool STAR = sizeof(T) == 8 && BEH == BEH_EP;
+    constexpr bool FUSE = FIN && STAR;
     constexpr int C = START ? 12 : 6; // LDS columns per line
     constexpr int EO = START ? 6 : 0; // column of eps_0 within a line
     constexpr int W = TY * C;
@@ -61,7 +65,7 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
     constexpr int NT = ZNT;
     constexpr int NW = NT / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[NBSUM][NW];
+    __shared__ double red[NBSUM + 2][NW];
     cx<T>* sl = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw = sl + NZH * WS;
 
@@ -80,7 +84,7 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
     // one voxel pair per thread; the iterations (PRE) load the pair's J2
     // history and phases before the transforms, so that they arrive during them
     static_assert(TY * LZ <= NT, "one voxel pair per thread");
-    constexpr bool PRE = STAR && !START;
+    constexpr bool PRE = STAR;
     T hs[12], hep[2];
     int hph[2] = {0, 0};
     if (PRE && tid < TY * LZ) {
@@ -130,9 +134,9 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
     }
 
     // constitutive update per voxel pair
-    double acc[NBSUM];
+    double acc[NBSUM + 2]; // sigma, eps; FUSE: sigma_eff, eps_p
 #pragma unroll
-    for (int q = 0; q < NBSUM; ++q) acc[q] = 0.0;
+    for (int q = 0; q < NBSUM + 2; ++q) acc[q] = 0.0;
     bool bad = false;
     for (int q = tid; q < TY * LZ; q += NT) {
         const int yl = q / LZ, m = q - yl * LZ;
@@ -145,7 +149,7 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
                 e2[c] = u.re;
                 e2[6 + c] = u.im;
             }
-            if (STORE) fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
+            if (STORE && !FUSE) fast::vstore<T, 12>(eps + (long long)v0 * 6, e2);
         } else {
             fast::vload<T, 12>(eps + (long long)v0 * 6, e2);
         }
@@ -158,24 +162,37 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
             if (BEH == BEH_EP) {
                 T so[6], eo[6], phi, dep, seff;
                 if constexpr (PRE) {
-                    // sigma_star = sigma_old - C eps_old, written by this increment's start
+                    // sigma_star = sigma_old - C eps_old of this increment
 #pragma unroll
                     for (int c = 0; c < 6; ++c) so[c] = hs[6 * h + c];
                     if (!j2_update<T, LIN, true>(mt, e, eo, so, hep[h], sg, phi, dep, seff)) bad = true;
                 } else {
                     fast::vload<T, 6>(p.sig_old + vg * 6, so);
                     fast::vload<T, 6>(p.eps_old + vg * 6, eo);
-                    if constexpr (STAR) {
-                        const double la = mt[0], mu = mt[1];
-                        T ce[6], st[6];
-                        iso_C(T(la), T(mu), eo, ce);
-#pragma unroll
-                        for (int c = 0; c < 6; ++c) st[c] = so[c] - ce[c];
-                        fast::vstore<T, 6>(p.sig_star + vg * 6, st);
-                    }
                     if (!j2_update<T, LIN>(mt, e, eo, so, p.ep[vg], sg, phi, dep, seff)) bad = true;
                 }
-                if (STORE) {
+                if constexpr (FUSE) {
+                    // export sums of this iterate, then update_old_Variable
+                    acc[NBSUM] += (double)seff;
+                    acc[NBSUM + 1] += (double)hep[h];
+                    p.seff[vg] = seff;
+                    p.ep[vg] = phi > T(0) ? hep[h] + dep : hep[h];
+                    T eprev[6], en[6], ek[6], sk[6], st[6], ce[6];
+                    fast::vload<T, 6>(p.eps_old + vg * 6, eprev);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        ek[c] = e[c];
+                        sk[c] = sg[c];
+                        en[c] = e[c] + (e[c] - eprev[c]);
+                    }
+                    iso_C(T(mt[0]), T(mt[1]), ek, ce);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) st[c] = sk[c] - ce[c];
+                    fast::vstore<T, 6>(p.eps + vg * 6, en);
+                    fast::vstore<T, 6>(p.eps_old + vg * 6, ek);
+                    fast::vstore<T, 6>(p.sig_old + vg * 6, sk);
+                    fast::vstore<T, 6>(p.sig_star + vg * 6, st);
+                } else if (STORE) {
                     p.phi[vg] = phi;
                     p.dep[vg] = dep;
                     p.seff[vg] = seff;
@@ -201,23 +218,31 @@ __global__ void __launch_bounds__(ZNT) M3D_BZ_ATTR kzb(Geo g, BPtr<T> p)
         }
     }
     if (bad) ctl.err = 1;
-    if (FIN) return;
-    // fixed-order block sums of the 12 means
+    if (FIN && !FUSE) return;
+    // fixed-order block sums of the 12 means (FUSE: the 14 export sums, in
+    // the order eps, sigma, sigma_eff, eps_p)
     {
+        constexpr int NQ = FUSE ? NBSUM + 2 : NBSUM;
         const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
-        for (int q = 0; q < NBSUM; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const double v = wave_sum(acc[q]);
             if (lane == 0) red[q][wid] = v;
         }
         __syncthreads();
-        if (tid < NBSUM) {
+        if (tid < NQ) {
             double r = 0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) r += red[tid][w];
-            p.psum[(long long)tid * g.nbz + t] = r;
+            if constexpr (FUSE) {
+                const int o = tid < 6 ? tid + 6 : (tid < 12 ? tid - 6 : tid); // sigma_c, eps_c -> eps_c, sigma_c
+                p.pfin[(long long)o * g.nbz + t] = r;
+            } else {
+                p.psum[(long long)tid * g.nbz + t] = r;
+            }
         }
     }
+    if constexpr (FUSE) return;
     __syncthreads();
 
     // r2c of the W lines

@@ -43,6 +43,7 @@ struct BPtr {
     T* dep;          // N   plastic multiplier increment ("ep")
     T* seff;         // N   von Mises stress after the return map ("ep")
     T* sig_star;     // 6N  sigma_old - C eps_old, once per increment ("ep", FP64 specialised kernels)
+    double* pfin;    // [14][nbz] block sums of the fused FINAL pass ("ep", FP64 specialised kernels)
     long long sU;    // per-solve stride of the 6N fields
     long long sN;    // per-solve stride of the N fields
     cx<T>* S6;       // Green output, 6 components (read by the z pass)
@@ -619,6 +620,35 @@ __global__ void k_bupdate(Geo g, BPtr<T> p, int nsolve)
     }
 }
 
+// the 14 export sums of the fused FINAL pass: its block sums pfin[q][nbz],
+// added in nblk contiguous shares (fixed order), out[q][b] as k_bsums writes
+template <int U = 0> // a template so that every unit including this header may instantiate it
+__global__ void __launch_bounds__(256) k_bfsums(Geo g, const double* pfin, double* out, int nblk)
+{
+    __shared__ double red[256 / 64];
+    const int q = blockIdx.y, b = blockIdx.x;
+    const int per = (g.nbz + nblk - 1) / nblk;
+    const int i0 = b * per, i1 = min(g.nbz, i0 + per);
+    double v = 0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) v += pfin[(long long)q * g.nbz + i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) out[(long long)q * nblk + b] = v;
+}
+
+// sigma_star = sigma_old - C eps_old for the next increment's start stage on
+// the specialised FP64 "ep" kernels, after k_bupdate (one solve)
+template <typename T>
+__global__ void k_bstar(Geo g, BPtr<T> p)
+{
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < g.N; v += (long long)gridDim.x * blockDim.x) {
+        const double* mt = p.mat + p.ph[v] * 8;
+        T eo[6], ce[6];
+        for (int c = 0; c < 6; ++c) eo[c] = p.eps_old[v * 6 + c];
+        iso_C(T(mt[0]), T(mt[1]), eo, ce);
+        for (int c = 0; c < 6; ++c) p.sig_star[v * 6 + c] = p.sig_old[v * 6 + c] - ce[c];
+    }
+}
+
 // copy fields between the FP64 and FP32 states of a solve; what: 1 eps,
 // 2 sig, 4 sig_old, 8 eps_old, 16 ep|d, 32 phi, dep and seff, 64 the Green
 // output spectrum S6 (6 x H complex)
@@ -657,6 +687,7 @@ __global__ void k_binit(Geo g, BPtr<T> p, int nsolve, int beh)
                 p.eps[i6 + k] = T(c.E[k]); // setLoad(dE, X0) (:799)
                 p.eps_old[i6 + k] = T(0);
                 if (beh == BEH_EP) p.sig_old[i6 + k] = T(0);
+                if (beh == BEH_EP && p.sig_star) p.sig_star[i6 + k] = T(0); // sigma_old - C eps_old
             }
             p.ep[i1] = beh == BEH_EP ? T(.000000001236547) : T(0); // :67-69 / :89-90
         }

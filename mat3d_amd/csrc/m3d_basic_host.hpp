@@ -34,6 +34,7 @@ struct BasicRunner {
     size_t neh = 0;         // complex values of Eh
     double* psum = nullptr;
     double* psum2 = nullptr; // [NBSUM][BSPLIT] shares of psum
+    double* pfin = nullptr;  // [14][nbz] block sums of the fused FINAL pass
     bool lin = false;        // every phase has hardening exponent n = 1
     double* pchk = nullptr;
     double* bs = nullptr;
@@ -88,6 +89,7 @@ struct BasicRunner {
             }
             e.alloc(&psum, sizeof(double) * NBSUM * nbz);
             e.alloc(&psum2, sizeof(double) * NBSUM * BSPLIT);
+            if (fast) e.alloc(&pfin, sizeof(double) * 14 * (size_t)gf.nbz);
             e.alloc(&pchk, sizeof(double) * (nbx + 1));
             e.alloc(&bs, sizeof(double) * 14 * nblk);
             e.alloc(&mat, sizeof(double) * 256 * 8);
@@ -107,10 +109,10 @@ struct BasicRunner {
     }
     void release()
     {
-        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)Eh, (void*)psum, (void*)psum2, (void*)pchk,
+        for (void* q : {(void*)f6, (void*)f1, (void*)S6, (void*)S12, (void*)Eh, (void*)psum, (void*)psum2, (void*)pfin, (void*)pchk,
                         (void*)bs, (void*)mat, (void*)ctl, (void*)f6s, (void*)f1s, (void*)S6s, (void*)S12s, (void*)Ehs})
             if (q) (void)hipFree(q);
-        f6 = f1 = psum = psum2 = pchk = bs = mat = nullptr;
+        f6 = f1 = psum = psum2 = pfin = pchk = bs = mat = nullptr;
         S6 = S12 = Eh = nullptr;
         f6s = f1s = nullptr;
         S6s = S12s = Ehs = nullptr;
@@ -141,6 +143,7 @@ struct BasicRunner {
         p.sig_old = a6 + 12 * N;
         p.eps_old = a6 + 18 * N;
         p.sig_star = sizeof(T) == 8 ? a6 + 24 * N : nullptr;
+        p.pfin = sizeof(T) == 8 ? pfin : nullptr;
         p.ep = a1;
         p.phi = a1 + N;
         p.dep = a1 + 2 * N;
@@ -248,7 +251,9 @@ struct BasicRunner {
 
     // the fields and state of the increment's last iterate on the
     // specialised FP64 path, whose iterations do not store them: the z pass
-    // in FINAL mode on the S6 buffer that iterate was computed from.  The
+    // in FINAL mode on the S6 buffer that iterate was computed from (for
+    // "ep" it also leaves the export sums in pfin and applies
+    // update_old_Variable).  The
     // iterations after the start stage (itNo it0) alternate the buffers from
     // buffer 0, so iterate itNo c.it read buffer (c.it - it0 - 1) & 1.  The
     // z pass skips a solve marked done, so the flag is cleared around it; c
@@ -357,6 +362,7 @@ struct BasicRunner {
             c.it = 0;
             c.err = 0;
             bool fp64_run = true;
+            bool fused = false; // the FINAL pass (FP64 "ep") ended the increment
             if (two) {
                 // FP32 phase from the increment's FP64 start state
                 c.tol_f = std::max(tol_f * e.polish_factor, e.fp32_floor);
@@ -388,7 +394,10 @@ struct BasicRunner {
                     stage_fast<double>(BMODE_RESTART, 1);
                     HCHK(hipGetLastError());
                     replay(g64.x);
-                    if (c.done != 3) final_fast(c, it0);
+                    if (c.done != 3) {
+                        final_fast(c, it0);
+                        fused = beh == BEH_EP;
+                    }
                     fp64_run = false;
                 } else {
                     // the FP32 return map failed: redo the increment in FP64 (its
@@ -405,7 +414,10 @@ struct BasicRunner {
                 run_stage(BMODE_START, 1);
                 HCHK(hipGetLastError());
                 replay(g64.x);
-                if (fast && c.done != 3) final_fast(c, 0);
+                if (fast && c.done != 3) {
+                    final_fast(c, 0);
+                    fused = beh == BEH_EP;
+                }
             }
             if (c.done == 3)
                 throw ArgError{M3D_ERR_NUMERIC, "constitutive update: Newton iteration failed (NaN or no convergence)"};
@@ -415,8 +427,11 @@ struct BasicRunner {
             info.eqVar = c.eqvar;
             info.diff_sum = c.dsum;
             info.diff_eq = c.deq;
-            // export_homogen_var(inc) (:722-746)
-            if (beh == BEH_EP)
+            // export_homogen_var(inc) (:722-746): from the fused FINAL pass's
+            // block sums, or from the fields
+            if (fused)
+                hipLaunchKernelGGL((k_bfsums<0>), dim3(nblk, 14), dim3(256), 0, st, gf, (const double*)pfin, bs, nblk);
+            else if (beh == BEH_EP)
                 hipLaunchKernelGGL((k_bsums<double, BEH_EP>), dim3(nblk, 1), dim3(256), 0, st, g12, bp, bs, nblk);
             else
                 hipLaunchKernelGGL((k_bsums<double, BEH_ED>), dim3(nblk, 1), dim3(256), 0, st, g12, bp, bs, nblk);
@@ -444,11 +459,16 @@ struct BasicRunner {
                 exit_inc = inc;
                 break;
             }
-            // update_old_Variable (:693-719); E += dE does not feed back (updateE recomputes E)
-            if (beh == BEH_EP)
-                hipLaunchKernelGGL((k_bupdate<double, BEH_EP>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
-            else
-                hipLaunchKernelGGL((k_bupdate<double, BEH_ED>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
+            // update_old_Variable (:693-719); E += dE does not feed back (updateE recomputes E).
+            // The fused FINAL pass has applied it; else here, and on the specialised
+            // kernels sigma_star for the next start stage
+            if (!fused) {
+                if (beh == BEH_EP)
+                    hipLaunchKernelGGL((k_bupdate<double, BEH_EP>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
+                else
+                    hipLaunchKernelGGL((k_bupdate<double, BEH_ED>), dim3(8192), dim3(256), 0, st, g12, bp, 1);
+                if (beh == BEH_EP && fast) hipLaunchKernelGGL((k_bstar<double>), dim3(8192), dim3(256), 0, st, g12, bp);
+            }
         }
         HCHK(hipEventRecord(e.ev1, st));
         HCHK(hipStreamSynchronize(st));
