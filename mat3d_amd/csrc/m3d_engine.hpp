@@ -802,7 +802,7 @@ struct Engine {
         } else {
             zpass<T, MODE_FWD>(g, p);
         }
-        hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.psum);
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
         ypass<T, 1, BUF_MAIN>(g, p);
@@ -822,14 +822,14 @@ struct Engine {
         } else {
             zpass<T, MODE_IUF>(g, p);
         }
-        hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.psum);
+        hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.psum);
         zpass<T, MODE_CHK_CUR>(g, p);
         ypass<T, -1, BUF_CHK_CUR>(g, p);
         xcheck<T, BUF_CHK_CUR>(g, p);
         zpass<T, MODE_CHK_PREV>(g, p);
         ypass<T, -1, BUF_CHK_PREV>(g, p);
         xcheck<T, BUF_CHK_PREV>(g, p);
-        hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(FTHREADS), 0, st, g, p.ctl, p.pchk);
+        hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.pchk);
         if (!green) return;
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
@@ -850,7 +850,7 @@ struct Engine {
     void launch_start_corr(const Geo& gf, const Ptr<float>& pf)
     {
         zpass<float, MODE_FWD>(gf, pf);
-        hipLaunchKernelGGL((k_fin0<C>), dim3(gf.nsolve), dim3(FTHREADS), 0, st, gf, pf.ctl, pf.psum);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(gf.nsolve), dim3(ATHREADS), 0, st, gf, pf.ctl, pf.psum);
         ypass<float, -1, BUF_MAIN>(gf, pf);
         xgreen<float>(gf, pf);
         ypass<float, 1, BUF_MAIN>(gf, pf);
@@ -859,14 +859,14 @@ struct Engine {
     void launch_iter_corr(const Geo& gf, const Ptr<float>& pf, const Geo& gd, const Ptr<double>& pd)
     {
         zpass<float, MODE_IUFR>(gf, pf);
-        hipLaunchKernelGGL((k_finA<C>), dim3(gf.nsolve), dim3(FTHREADS), 0, st, gf, pf.ctl, pf.psum);
+        hipLaunchKernelGGL((k_finA<C>), dim3(gf.nsolve), dim3(ATHREADS), 0, st, gf, pf.ctl, pf.psum);
         zpass<double, MODE_CHK_CUR>(gd, pd);
         ypass<double, -1, BUF_CHK_CUR>(gd, pd);
         xcheck<double, BUF_CHK_CUR>(gd, pd);
         zpass<double, MODE_CHK_PREV>(gd, pd);
         ypass<double, -1, BUF_CHK_PREV>(gd, pd);
         xcheck<double, BUF_CHK_PREV>(gd, pd);
-        hipLaunchKernelGGL((k_finB<C>), dim3(gd.nsolve), dim3(FTHREADS), 0, st, gd, pd.ctl, pd.pchk);
+        hipLaunchKernelGGL((k_finB<C>), dim3(gd.nsolve), dim3(ATHREADS), 0, st, gd, pd.ctl, pd.pchk);
         ypass<float, -1, BUF_MAIN>(gf, pf);
         xgreen<float>(gf, pf);
         ypass<float, 1, BUF_MAIN>(gf, pf);

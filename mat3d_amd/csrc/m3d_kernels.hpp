@@ -792,19 +792,37 @@ __global__ void __launch_bounds__(XYTHREADS) k_xcheck(Geo g, Ptr<T> p)
 
 // ---------------------------------------------------------- finalizers
 constexpr int FTHREADS = 256;
+// the accelerated scheme's finalisers: one workgroup per solve over up to
+// Nx*Ny/TY (z pass) or Ny*Nzh/TK (check x pass) partial sums, 65536 at 512^3
+constexpr int ATHREADS = 1024;
 
+// sum of a[0..n) by one workgroup in a fixed order: every thread keeps 8
+// partial sums of its strided elements (8 loads in flight, not one chain of
+// dependent adds), combined pairwise, then the workgroup's tree
 __device__ __forceinline__ double ordered_sum(const double* a, int n, double* red)
 {
-    double v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += a[i];
-    return block_sum(v, red);
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0, p6 = 0, p7 = 0;
+    const int st = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 7 * st < n; i += 8 * st) {
+        p0 += a[i];
+        p1 += a[i + st];
+        p2 += a[i + 2 * st];
+        p3 += a[i + 3 * st];
+        p4 += a[i + 4 * st];
+        p5 += a[i + 5 * st];
+        p6 += a[i + 6 * st];
+        p7 += a[i + 7 * st];
+    }
+    for (; i < n; i += st) p0 += a[i];
+    return block_sum(((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)), red);
 }
 
 // after the forward-only z pass: sum of the dual field at the start iterate
 template <int C>
-__global__ void __launch_bounds__(FTHREADS) k_fin0(Geo g, Ctl* ctl, const double* psum)
+__global__ void __launch_bounds__(ATHREADS) k_fin0(Geo g, Ctl* ctl, const double* psum)
 {
-    __shared__ double red[FTHREADS / 64];
+    __shared__ double red[ATHREADS / 64];
     const int s = blockIdx.x;
     Ctl& c = ctl[s];
     const int done = c.done;
@@ -827,9 +845,9 @@ __global__ void __launch_bounds__(FTHREADS) k_fin0(Geo g, Ctl* ctl, const double
 // after the update pass: itNo += 1, diff_sum_field, decide whether the
 // equilibrium check is needed (FFT_analysis3D.cpp:392-416, stop rule :785-792)
 template <int C>
-__global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double* psum)
+__global__ void __launch_bounds__(ATHREADS) k_finA(Geo g, Ctl* ctl, const double* psum)
 {
-    __shared__ double red[FTHREADS / 64];
+    __shared__ double red[ATHREADS / 64];
     const int s = blockIdx.x;
     Ctl& c = ctl[s];
     const int done = c.done;
@@ -851,9 +869,9 @@ __global__ void __launch_bounds__(FTHREADS) k_finA(Geo g, Ctl* ctl, const double
 
 // after the check FFTs: eqVar, diff_avr_eqVar and the stop decision
 template <int C>
-__global__ void __launch_bounds__(FTHREADS) k_finB(Geo g, Ctl* ctl, const double* pchk)
+__global__ void __launch_bounds__(ATHREADS) k_finB(Geo g, Ctl* ctl, const double* pchk)
 {
-    __shared__ double red[FTHREADS / 64];
+    __shared__ double red[ATHREADS / 64];
     const int s = blockIdx.x;
     Ctl& c = ctl[s];
     const int done = c.done, need_cur = c.need_cur, need_prev = c.need_prev;
