@@ -974,7 +974,7 @@ struct Engine {
                     tune32.xg_wide = vars[v][1];
                     tune32.xg_rp = vars[v][2];
                     const float t = time_it([&]() { xgreen<float>(g, p); });
-                    if (r > 0) bt[v] = std::min(bt[v], t); // round 0 warms every variant up
+                    if (r > 0) bt[v] = std::min(bt[v], t); // the first timing pass warms every variant up
                 }
             const size_t k = std::min_element(bt.begin(), bt.end()) - bt.begin();
             best.xg_nt = vars[k][0];

@@ -57,13 +57,12 @@ def test_thermal12_against_fixture(precision, tol):
     assert np.linalg.norm(s.KHomo - f["n3_K"]) / np.linalg.norm(f["n3_K"]) < tol
 
 
-def test_cpp_compat_example_runs():
+def test_cpp_compat_example_runs(tmp_path):
     """The C++ caller (include/m3d_compat.hpp) drives the same solve."""
     f = load("elastic12.npz")
     exe = os.path.join(ROOT, "mat3d_amd", "examples", "homogenize")
     assert os.path.exists(exe), "build() compiles the example"
-    rve = os.path.join(ROOT, "gpurun_out", "rve12.txt")
-    os.makedirs(os.path.dirname(rve), exist_ok=True)
+    rve = str(tmp_path / "rve12.txt")
     np.savetxt(rve, f["ids"].reshape(-1), fmt="%d")
     mat = f["stress_mat"]
     out = subprocess.run([exe, "12", rve, *(f"{float(v):.17g}" for v in (mat[0, 0], mat[0, 1], mat[1, 0], mat[1, 1])),

@@ -2,7 +2,7 @@
 
 Bars: FP64 path -- same iteration counts, C_eff within 1e-9 relative, fields
 within 1e-9 relative; FP32+FP64-polish path -- C_eff within 1e-6 relative
-voxel stress within 1e-5 relative (max-abs over max-abs).
+(the bar of the FP32 path), voxel stress within 1e-5 relative (max-abs over max-abs).
 """
 import numpy as np
 import pytest
