@@ -660,6 +660,12 @@ struct Engine {
         else hipLaunchKernelGGL((k_zpass<T, PH, GM, false>), grid, dim3(ZTHREADS), zsm, st, g, p);
     }
 
+    // threads of a finaliser workgroup over n partial sums per solve: the
+    // wide form where the loads dominate (large grids); 256 threads
+    // otherwise, since a 1024-thread workgroup waits for a whole free CU
+    // while the kernels of other engines' streams share the GPU
+    static unsigned fin_threads(long long n) { return n >= 8192 ? ATHREADS : FTHREADS; }
+
     // workgroups of a launch: one per tile, or for the check passes (which
     // loop over their tiles) at most M3D_CHK_WG (PH::chk_grid by default)
     static unsigned grid_of(long long tiles, bool check)
@@ -802,7 +808,7 @@ struct Engine {
         } else {
             zpass<T, MODE_FWD>(g, p);
         }
-        hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.psum);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(g.nsolve), dim3(fin_threads(g.nbz)), 0, st, g, p.ctl, p.psum);
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
         ypass<T, 1, BUF_MAIN>(g, p);
@@ -822,14 +828,14 @@ struct Engine {
         } else {
             zpass<T, MODE_IUF>(g, p);
         }
-        hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.psum);
+        hipLaunchKernelGGL((k_finA<C>), dim3(g.nsolve), dim3(fin_threads(g.nbz)), 0, st, g, p.ctl, p.psum);
         zpass<T, MODE_CHK_CUR>(g, p);
         ypass<T, -1, BUF_CHK_CUR>(g, p);
         xcheck<T, BUF_CHK_CUR>(g, p);
         zpass<T, MODE_CHK_PREV>(g, p);
         ypass<T, -1, BUF_CHK_PREV>(g, p);
         xcheck<T, BUF_CHK_PREV>(g, p);
-        hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(ATHREADS), 0, st, g, p.ctl, p.pchk);
+        hipLaunchKernelGGL((k_finB<C>), dim3(g.nsolve), dim3(fin_threads(g.nbc)), 0, st, g, p.ctl, p.pchk);
         if (!green) return;
         ypass<T, -1, BUF_MAIN>(g, p);
         xgreen<T>(g, p);
@@ -850,7 +856,7 @@ struct Engine {
     void launch_start_corr(const Geo& gf, const Ptr<float>& pf)
     {
         zpass<float, MODE_FWD>(gf, pf);
-        hipLaunchKernelGGL((k_fin0<C>), dim3(gf.nsolve), dim3(ATHREADS), 0, st, gf, pf.ctl, pf.psum);
+        hipLaunchKernelGGL((k_fin0<C>), dim3(gf.nsolve), dim3(fin_threads(gf.nbz)), 0, st, gf, pf.ctl, pf.psum);
         ypass<float, -1, BUF_MAIN>(gf, pf);
         xgreen<float>(gf, pf);
         ypass<float, 1, BUF_MAIN>(gf, pf);
@@ -859,14 +865,14 @@ struct Engine {
     void launch_iter_corr(const Geo& gf, const Ptr<float>& pf, const Geo& gd, const Ptr<double>& pd)
     {
         zpass<float, MODE_IUFR>(gf, pf);
-        hipLaunchKernelGGL((k_finA<C>), dim3(gf.nsolve), dim3(ATHREADS), 0, st, gf, pf.ctl, pf.psum);
+        hipLaunchKernelGGL((k_finA<C>), dim3(gf.nsolve), dim3(fin_threads(gf.nbz)), 0, st, gf, pf.ctl, pf.psum);
         zpass<double, MODE_CHK_CUR>(gd, pd);
         ypass<double, -1, BUF_CHK_CUR>(gd, pd);
         xcheck<double, BUF_CHK_CUR>(gd, pd);
         zpass<double, MODE_CHK_PREV>(gd, pd);
         ypass<double, -1, BUF_CHK_PREV>(gd, pd);
         xcheck<double, BUF_CHK_PREV>(gd, pd);
-        hipLaunchKernelGGL((k_finB<C>), dim3(gd.nsolve), dim3(ATHREADS), 0, st, gd, pd.ctl, pd.pchk);
+        hipLaunchKernelGGL((k_finB<C>), dim3(gd.nsolve), dim3(fin_threads(gd.nbc)), 0, st, gd, pd.ctl, pd.pchk);
         ypass<float, -1, BUF_MAIN>(gf, pf);
         xgreen<float>(gf, pf);
         ypass<float, 1, BUF_MAIN>(gf, pf);
