@@ -212,9 +212,10 @@ def main():
     if not args.no_cpu:
         # the box's CPU share for one GPU is 16 threads, whatever nproc shows
         threads = min(16, os.cpu_count() or 1)
-        v, tcpu, done = cpu_baseline(ids, mat, n, threads)
+        # samples of about 10 s each: 15 iterations on every thread, 3 with the reference's threading
+        v, tcpu, done = cpu_baseline(ids, mat, n, threads, iters=15)
         print(f"[bench] cpu oracle: {v:.4e} voxel-it/s on {threads} threads", flush=True)
-        vr, tr, dr = cpu_baseline(ids, mat, n, 1, iters=1, fft_threads=min(3, threads))
+        vr, tr, dr = cpu_baseline(ids, mat, n, 1, iters=3, fft_threads=min(3, threads))
         print(f"[bench] cpu oracle, reference threading (3 FFT threads, serial voxel loops): {vr:.4e} voxel-it/s",
               flush=True)
         cpu = {"value": v, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
